@@ -1,0 +1,145 @@
+/*
+ * tmvx.h -- C ABI of the MI355X (gfx950) dense GEMM + LU library.
+ *
+ * This is the drop-in seam for TMV 0.76's dense GEMM and LU path.  TMV's own
+ * external seam is BLAS/LAPACK shaped (dgemm / dgetrf / dgetrs / dtrsm,
+ * src/TMV_MultMM.cpp:102-127, src/TMV_LUDecompose.cpp:418-442,
+ * src/TMV_LUDiv.cpp:58-92, src/TMV_TriDiv_M.cpp:268-310); these entry points
+ * take the same arguments, with TMV's semantics where the two differ
+ * (0-based pivots, interchange lists as TMV's Permutation stores them).
+ *
+ * Conventions
+ *   - Matrices are column-major with a leading dimension unless a layout
+ *     argument says row-major.  All matrix/vector pointers are DEVICE
+ *     pointers; `stream` is a hipStream_t (NULL = default stream).
+ *   - Pivots are int64 (ptrdiff_t), 0-based: piv[j] is the row interchanged
+ *     with row j at step j (TMV LU_Decompose, include/tmv/TMV_LUD.h:78-79).
+ *   - Calls are asynchronous on `stream` unless documented otherwise; a
+ *     host-side out-parameter (e.g. `info`) means the call synchronises.
+ *   - Status codes are returned, never exceptions.  The library holds no
+ *     mutable global state other than per-device caches set up once.
+ */
+#ifndef TMVX_H
+#define TMVX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum tmvx_status {
+    TMVX_OK = 0,
+    TMVX_ERR_ARG = -1,          /* invalid argument (LAPACK info < 0)       */
+    TMVX_ERR_ALLOC = -2,        /* device allocation failed                 */
+    TMVX_ERR_HIP = -3,          /* HIP runtime / launch error               */
+    TMVX_ERR_SINGULAR = -4,     /* zero diagonal met a non-zero RHS entry   */
+    TMVX_ERR_NOT_CONVERGED = -5,/* iterative refinement missed its target   */
+    TMVX_ERR_TIMEOUT = -6,      /* a bounded device-side wait expired       */
+    TMVX_ERR_UNSUPPORTED = -7   /* valid request this build does not handle */
+};
+
+enum tmvx_layout { TMVX_COLMAJOR = 0, TMVX_ROWMAJOR = 1 };
+
+/* Human-readable text for a status code. */
+const char* tmvx_status_str(int status);
+
+/* Library version string. */
+const char* tmvx_version(void);
+
+/*
+ * C := alpha * op(A) * op(B) + beta * C, C column-major (m x n).
+ * layout_a / layout_b give the storage of A (m x k) and B (k x n):
+ * TMVX_COLMAJOR (lda >= m resp. ldb >= k) or TMVX_ROWMAJOR (lda >= k resp.
+ * ldb >= n).  TMV's MultMM<add>(alpha,A,B,C) is beta = add ? 1 : 0; when
+ * beta == 0, C is not read.  Layout pairs map to TMV's kernels:
+ * CC = CCCMultMM, CR = CRCMultMM, RC = RCCMultMM, RR = RCC after copying B
+ * (src/TMV_MultMM.cpp:79-91).
+ * Replaces: tmv::MultMM<add> (include/tmv/TMV_MatrixArithFunc.h:53-57,
+ * src/TMV_MultMM.cpp:404-477) and its dgemm seam (src/TMV_MultMM.cpp:102-127).
+ */
+int tmvx_gemm_f64(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k,
+                  double alpha, const double* A, int64_t lda, const double* B,
+                  int64_t ldb, double beta, double* C, int64_t ldc,
+                  void* stream);
+int tmvx_gemm_f32(int layout_a, int layout_b, int64_t m, int64_t n, int64_t k,
+                  float alpha, const float* A, int64_t lda, const float* B,
+                  int64_t ldb, float beta, float* C, int64_t ldc,
+                  void* stream);
+
+/*
+ * In-place PA = LU with partial pivoting of the n x n column-major A.
+ * On return A holds unit-lower L below the diagonal and U on and above it;
+ * piv (device, n entries) holds 0-based interchanges.  Like TMV, a column
+ * whose largest candidate underflows (|x| < DBL_MIN / FLT_MIN) is zeroed
+ * below the diagonal with piv[j] = j and the factorisation continues
+ * (src/TMV_LUDecompose.cpp:162-169, 286-292, 359-373); *info (host) gets
+ * the 1-based index of the first zero pivot, or 0.  Synchronises.
+ * Replaces: tmv::LU_Decompose(MatrixView<T>, ptrdiff_t*)
+ * (include/tmv/TMV_LUD.h:78-79, src/TMV_LUDecompose.cpp:514-559) and its
+ * dgetrf seam (src/TMV_LUDecompose.cpp:418-442).
+ */
+int tmvx_getrf_f64(int64_t n, double* A, int64_t lda, int64_t* piv,
+                   int64_t* info, void* stream);
+int tmvx_getrf_f32(int64_t n, float* A, int64_t lda, int64_t* piv,
+                   int64_t* info, void* stream);
+
+/*
+ * Solve with the factors of tmvx_getrf_*: trans == 0 solves A X = B
+ * (permute B's rows by piv, then unit-lower L, then U: TMV LU_LDivEq,
+ * src/TMV_LUDiv.cpp:185-226); trans != 0 solves A^T X = B (TMV LU_RDivEq on
+ * B^T, src/TMV_LUDiv.cpp:369-410).  B is n x nrhs column-major, overwritten
+ * with X.  A zero U diagonal met by a non-zero entry reports
+ * TMVX_ERR_SINGULAR (TMV throws SingularUpperTriMatrix,
+ * src/TMV_TriDiv_V.cpp:171-217); check_singular = 0 skips that check and
+ * the synchronisation it needs.
+ * Replaces: tmv::LU_LDivEq / LU_RDivEq (src/TMV_LUDiv.h:29-35) and the
+ * dgetrs seam (src/TMV_LUDiv.cpp:58-92).
+ */
+int tmvx_getrs_f64(int trans, int64_t n, int64_t nrhs, const double* LU,
+                   int64_t lda, const int64_t* piv, double* B, int64_t ldb,
+                   int check_singular, void* stream);
+int tmvx_getrs_f32(int trans, int64_t n, int64_t nrhs, const float* LU,
+                   int64_t lda, const int64_t* piv, float* B, int64_t ldb,
+                   int check_singular, void* stream);
+
+/*
+ * B := op(A)^-1 B with A n x n triangular (upper != 0: upper), unit != 0
+ * for an implicit unit diagonal, trans != 0 for A^T.  B is n x nrhs.
+ * Replaces: tmv::TriLDivEq(Tri, MatrixView) (include/tmv/TMV_TriDiv.h:72-75,
+ * src/TMV_TriDiv_M.cpp:190-266) and the dtrsm seam (src/TMV_TriDiv_M.cpp:
+ * 268-310).
+ */
+int tmvx_trsm_f64(int upper, int trans, int unit, int64_t n, int64_t nrhs,
+                  const double* A, int64_t lda, double* B, int64_t ldb,
+                  void* stream);
+int tmvx_trsm_f32(int upper, int trans, int unit, int64_t n, int64_t nrhs,
+                  const float* A, int64_t lda, float* B, int64_t ldb,
+                  void* stream);
+
+/*
+ * Apply interchanges piv[k1..k2) in order (forward != 0) or in reverse
+ * order to the rows of the ncols columns of A.  TMV permuteRows /
+ * reversePermuteRows (src/TMV_Matrix.cpp:865-930).
+ */
+int tmvx_laswp_f64(int forward, int64_t ncols, double* A, int64_t lda,
+                   int64_t k1, int64_t k2, const int64_t* piv, void* stream);
+int tmvx_laswp_f32(int forward, int64_t ncols, float* A, int64_t lda,
+                   int64_t k1, int64_t k2, const int64_t* piv, void* stream);
+
+/*
+ * Synthetic inputs: out[i] = lo + (hi-lo) * u(seed, offset + i), u in [0,1)
+ * from the counter-based SplitMix64 stream documented in tmv_amd/rng.py
+ * (bit-identical to it).  normal: Box-Muller over pairs of that stream.
+ * Not part of TMV; the benchmark's data generator.
+ */
+int tmvx_fill_uniform_f64(uint64_t seed, uint64_t offset, int64_t count,
+                          double lo, double hi, double* out, void* stream);
+int tmvx_fill_normal_f64(uint64_t seed, uint64_t offset, int64_t count,
+                         double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TMVX_H */

@@ -1,0 +1,241 @@
+"""The oracle: CPU restatement of TMV 0.76's dense GEMM + LU path.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
+CPU-baseline leg of bench.py import this package, and only as the checker.
+The product library (``tmv_amd``) never imports it and has no CPU fallback.
+
+Two native libraries back it:
+
+* ``oracle/liboracle.so`` -- ``tmv_oracle.cpp``, a fresh restatement of
+  TMV's algorithms (MultMM Block path, RecursiveLUDecompose, LU_LDivEq,
+  NonBlasTriLDivEq), built by :func:`build`.
+* ``oracle/_ref/libtmv_ref.so`` -- the real TMV library compiled from the
+  reference sources by ``oracle/Makefile.ref`` plus ``tmvref_driver.cpp``.
+  Present only where it was built (it travels with the tree as a binary).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libtmv_ref.so")
+REF_SRC = "/root/reference"
+
+_i64 = ctypes.c_int64
+_vp = ctypes.c_void_p
+
+
+def build(verbose: bool = False) -> None:
+    """Compile the restatement (always) and the real TMV (when its sources
+    are present in this container)."""
+    cmd = ["g++", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c++14",
+           os.path.join(HERE, "tmv_oracle.cpp"), "-o", ORACLE_SO + ".tmp"]
+    subprocess.run(cmd, check=True, capture_output=not verbose)
+    os.replace(ORACLE_SO + ".tmp", ORACLE_SO)
+    if os.path.isdir(os.path.join(REF_SRC, "src")):
+        root = os.path.dirname(HERE)
+        subprocess.run(["make", "-f", os.path.join("oracle", "Makefile.ref"),
+                        "REF=" + REF_SRC, "-j8"], cwd=root, check=True,
+                       capture_output=not verbose)
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class _Lib:
+    def __init__(self, path: str, prefix: str):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = ctypes.CDLL(path)
+        self.prefix = prefix
+        self.path = path
+
+    def fn(self, name: str):
+        return getattr(self.lib, self.prefix + name)
+
+
+_oracle = None
+_ref = None
+
+
+def oracle_lib() -> _Lib:
+    global _oracle
+    if _oracle is None:
+        _oracle = _Lib(ORACLE_SO, "orc_")
+    return _oracle
+
+
+def ref_lib() -> _Lib | None:
+    """The real TMV build, or None where it was not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_SO):
+        _ref = _Lib(REF_SO, "tmvref_")
+    return _ref
+
+
+_SUF = {np.dtype(np.float64): ("f64", ctypes.c_double),
+        np.dtype(np.float32): ("f32", ctypes.c_float),
+        np.dtype(np.longdouble): ("f80", ctypes.c_longdouble)}
+
+
+def _fortran(a, dtype):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def _fcopy(a, dtype):
+    return np.array(a, dtype=dtype, order="F", copy=True)
+
+
+def gemm(A, B, C=None, alpha=1.0, add=False, a_rowmajor=False,
+         b_rowmajor=False, dtype=np.float64, lib: _Lib | None = None):
+    """C (+)= alpha*A*B following tmv::MultMM<add>.  A and B are taken in the
+    stated storage order (row-major = C order), C is column-major."""
+    lib = lib or oracle_lib()
+    dtype = np.dtype(dtype)
+    suf, ct = _SUF[dtype]
+    A = np.ascontiguousarray(A, dtype) if a_rowmajor else _fortran(A, dtype)
+    B = np.ascontiguousarray(B, dtype) if b_rowmajor else _fortran(B, dtype)
+    m, k = A.shape
+    n = B.shape[1]
+    C = np.zeros((m, n), dtype, order="F") if C is None else _fcopy(C, dtype)
+    lda = A.shape[1] if a_rowmajor else max(1, m)
+    ldb = B.shape[1] if b_rowmajor else max(1, k)
+    f = lib.fn("gemm_" + suf)
+    f.argtypes = [ctypes.c_int, ctypes.c_int, _i64, _i64, _i64, ct, _vp, _i64,
+                  _vp, _i64, ctypes.c_int, _vp, _i64]
+    rc = f(int(a_rowmajor), int(b_rowmajor), m, n, k, alpha, _ptr(A), lda,
+           _ptr(B), ldb, int(add), _ptr(C), max(1, m))
+    if rc != 0:
+        raise RuntimeError(f"gemm failed: {rc}")
+    return C
+
+
+def getrf(A, dtype=np.float64, gaps=False, lib: _Lib | None = None):
+    """In-place PA = LU (TMV LU_Decompose).  Returns (LU, P) with P the
+    0-based interchange list; with gaps=True also the per-column relative
+    pivot gap (oracle restatement only)."""
+    lib = lib or oracle_lib()
+    dtype = np.dtype(dtype)
+    suf, _ = _SUF[dtype]
+    LU = _fcopy(A, dtype)
+    m, n = LU.shape
+    P = np.zeros(min(m, n), np.int64)
+    g = np.zeros(min(m, n), np.float64)
+    f = lib.fn("getrf_" + suf)
+    if lib.prefix == "orc_":
+        f.argtypes = [_i64, _i64, _vp, _i64, _vp, _vp]
+        rc = f(m, n, _ptr(LU), max(1, m), _ptr(P), _ptr(g) if gaps else None)
+    else:
+        if m != n:
+            raise ValueError("reference driver takes square matrices")
+        f.argtypes = [_i64, _vp, _i64, _vp]
+        rc = f(n, _ptr(LU), max(1, n), _ptr(P))
+    if rc != 0:
+        raise RuntimeError(f"getrf failed: {rc}")
+    return (LU, P, g) if gaps else (LU, P)
+
+
+class SingularError(RuntimeError):
+    """A zero diagonal met a non-zero right-hand side (TMV's Singular)."""
+
+
+def getrs(LU, P, B, trans=False, dtype=np.float64, lib: _Lib | None = None):
+    """Solve with the factors (TMV LU_LDivEq; trans -> LU_RDivEq)."""
+    lib = lib or oracle_lib()
+    dtype = np.dtype(dtype)
+    suf, _ = _SUF[dtype]
+    LU = _fortran(LU, dtype)
+    n = LU.shape[0]
+    B = np.asarray(B, dtype)
+    vec = B.ndim == 1
+    X = _fcopy(B.reshape(n, -1), dtype)
+    P = np.ascontiguousarray(P, np.int64)
+    f = lib.fn("getrs_" + suf)
+    f.argtypes = [ctypes.c_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64]
+    rc = f(int(trans), n, X.shape[1], _ptr(LU), max(1, n), _ptr(P), _ptr(X),
+           max(1, n))
+    if rc == -2:
+        raise SingularError("singular upper triangle")
+    if rc != 0:
+        raise RuntimeError(f"getrs failed: {rc}")
+    return X[:, 0] if vec else X
+
+
+def gesv(A, B, dtype=np.float64):
+    """x = b/A the TMV way: factor a copy, then solve."""
+    LU, P = getrf(A, dtype)
+    return getrs(LU, P, B, dtype=dtype)
+
+
+def ref_gesv(A, B, dtype=np.float64):
+    """x = b/A through the real TMV user API (divideUsing(LU))."""
+    lib = ref_lib()
+    if lib is None:
+        raise FileNotFoundError(REF_SO)
+    dtype = np.dtype(dtype)
+    suf, _ = _SUF[dtype]
+    A = _fortran(A, dtype)
+    n = A.shape[0]
+    Bv = np.asarray(B, dtype)
+    vec = Bv.ndim == 1
+    Bm = _fortran(Bv.reshape(n, -1), dtype)
+    X = np.zeros_like(Bm, order="F")
+    f = lib.fn("gesv_" + suf)
+    f.argtypes = [_i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64]
+    rc = f(n, Bm.shape[1], _ptr(A), n, _ptr(Bm), n, _ptr(X), n)
+    if rc == -2:
+        raise SingularError("singular")
+    if rc != 0:
+        raise RuntimeError(f"gesv failed: {rc}")
+    return X[:, 0] if vec else X
+
+
+def ref_det(A):
+    lib = ref_lib()
+    A = _fortran(A, np.float64)
+    out = ctypes.c_double(0.0)
+    f = lib.fn("det_f64")
+    f.argtypes = [_i64, _vp, _i64, ctypes.c_void_p]
+    rc = f(A.shape[0], _ptr(A), A.shape[0], ctypes.addressof(out))
+    if rc != 0:
+        raise RuntimeError("det failed")
+    return out.value
+
+
+def det_from_lu(LU, P) -> float:
+    f = oracle_lib().fn("det_from_lu_f64")
+    f.restype = ctypes.c_double
+    f.argtypes = [_i64, _vp, _i64, _vp]
+    LU = _fortran(LU, np.float64)
+    P = np.ascontiguousarray(P, np.int64)
+    return f(LU.shape[0], _ptr(LU), LU.shape[0], _ptr(P))
+
+
+def set_threads(n: int) -> None:
+    for lib in (oracle_lib(), ref_lib()):
+        if lib is not None:
+            lib.fn("set_threads")(ctypes.c_int(n))
+
+
+def unpack_lu(LU):
+    """(L unit-lower, U upper) from the packed factors."""
+    LU = np.asarray(LU)
+    n = min(LU.shape)
+    L = np.tril(LU[:, :n], -1) + np.eye(LU.shape[0], n, dtype=LU.dtype)
+    U = np.triu(LU[:n, :])
+    return L, U
+
+
+def apply_perm_rows(A, P):
+    """Apply the interchange list in order (TMV permuteRows)."""
+    A = np.array(A, copy=True)
+    for i, p in enumerate(P):
+        if p != i:
+            A[[i, p]] = A[[p, i]]
+    return A

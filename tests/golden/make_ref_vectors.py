@@ -1,0 +1,83 @@
+"""Write tests/golden/tmv_ref_vectors.npz: outputs of the real TMV 0.76
+library (built from the reference sources into oracle/_ref by
+oracle/Makefile.ref) on seeded inputs.
+
+    python tests/golden/make_ref_vectors.py
+
+Inputs come from tmv_amd.rng (SplitMix64), so they can be regenerated
+anywhere; only TMV's outputs are stored.  Large outputs are stored as
+SHA-256 digests of their column-major float64 bytes.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+
+import oracle  # noqa: E402
+from tmv_amd import rng  # noqa: E402
+import fixtures as F  # noqa: E402
+
+OUT = os.path.join(ROOT, "tests", "golden", "tmv_ref_vectors.npz")
+SEED = 0x7A15EED
+
+
+def digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.asfortranarray(a).tobytes(order="F")).hexdigest()
+
+
+def main() -> None:
+    ref = oracle.ref_lib()
+    if ref is None:
+        raise SystemExit("oracle/_ref/libtmv_ref.so is not built")
+    v = {}
+    # GEMM, all four operand layouts (A,B given column-major; the layout
+    # flag says how TMV views the same values).
+    A = rng.uniform_matrix(SEED + 101, 64, 80)
+    B = rng.uniform_matrix(SEED + 102, 80, 48)
+    for ar in (0, 1):
+        for br in (0, 1):
+            v[f"gemm_64x48x80_{ar}{br}"] = oracle.gemm(A, B, a_rowmajor=ar, b_rowmajor=br, lib=ref)
+    C0 = rng.uniform_matrix(SEED + 103, 64, 48)
+    v["gemm_64x48x80_add"] = oracle.gemm(A, B, C=C0, alpha=-0.5, add=True, lib=ref)
+    # LU on uniform [-1,1] inputs without a diagonal shift (pivoting active).
+    for n in (1, 2, 3, 5, 37, 64, 100, 130):
+        M = rng.uniform_matrix(SEED + 200 + n, n, n)
+        LU, P = oracle.getrf(M, lib=ref)
+        v[f"lu_{n}_LU"] = LU
+        v[f"lu_{n}_P"] = P
+        b = rng.uniform(SEED + 300 + n, n, -1.0, 1.0)
+        v[f"solve_{n}_x"] = oracle.ref_gesv(M, b)
+    # TMV's own fixtures (tests/TMV_TestMatrixDecomp.cpp): pivots + digest.
+    for t in (0, 3, 4, 5, 6):
+        LU, P = oracle.getrf(F.decomp_matrix(t), lib=ref)
+        v[f"decomp_{t}_P"] = P
+        v[f"decomp_{t}_LU"] = LU
+    LU, P = oracle.getrf(F.big_div_100(), lib=ref)
+    v["bigdiv_P"] = P
+    v["bigdiv_LU"] = LU
+    # The plain CPU-runnable case: 512^3 GEMM and an LU solve on A + 512 I.
+    n = 512
+    A = rng.uniform_matrix(SEED + 0, n, n)
+    B = rng.uniform_matrix(SEED + 1, n, n)
+    C = oracle.gemm(A, B, lib=ref)
+    v["cfg1_gemm_sha256"] = np.array(digest(C))
+    v["cfg1_gemm_colsum"] = C.sum(axis=0)
+    As = A + n * np.eye(n)
+    LU, P = oracle.getrf(As, lib=ref)
+    v["cfg1_lu_P"] = P
+    v["cfg1_lu_sha256"] = np.array(digest(LU))
+    b = rng.uniform(SEED + 2, n, -1.0, 1.0)
+    v["cfg1_x"] = oracle.ref_gesv(As, b)
+    np.savez_compressed(OUT, **v)
+    print(f"wrote {OUT}: {len(v)} arrays, {os.path.getsize(OUT)} bytes")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,161 @@
+"""LU factorisation and solves on the GPU through the C ABI vs the oracle
+(TMV LU_Decompose / LU_LDivEq restated).  Bar: identical pivots on
+tie-free inputs, ||P A - L U|| <= 8 n eps ||A||, solve backward error
+<= 8 n eps."""
+import numpy as np
+import pytest
+import torch
+
+import fixtures as F
+import tmv_amd
+from checks import EPS64, EPS32, lu_residual, backward_error
+from devutil import to_dev, to_host
+from tmv_amd import rng
+
+pytestmark = pytest.mark.gpu
+SEED = 0x7A15EED
+
+
+def gpu_getrf(A, dtype=torch.float64):
+    Ad = to_dev(A, dtype)
+    piv, info = tmv_amd.getrf(Ad)
+    return Ad, to_host(Ad), piv.cpu().numpy(), info
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 17, 63, 64, 65, 100, 130, 257, 512, 600, 1000])
+def test_getrf_pivots_and_residual(orc, dev, n):
+    print(f"getrf n={n}", flush=True)
+    A = rng.uniform_matrix(SEED + 400 + n, n, n)
+    LUo, Po, gaps = orc.getrf(A, gaps=True)
+    _, LU, P, info = gpu_getrf(A)
+    assert info == 0
+    ok = gaps > 64 * n * EPS64
+    k0 = n if ok.all() else int(np.argmin(ok))
+    assert np.array_equal(P[:k0], Po[:k0])
+    if n <= 600:
+        assert lu_residual(A, LU, P) <= 8 * n * EPS64
+
+
+@pytest.mark.parametrize("mattype", [0, 3, 4, 5, 6])
+def test_getrf_tmv_decomp_fixtures(orc, dev, mattype):
+    """TMV's own decomposition fixtures, incl. the singular, near-underflow
+    (zero-column rule) and near-overflow matrices."""
+    A = F.decomp_matrix(mattype)
+    LUo, Po, gaps = orc.getrf(A, gaps=True)
+    _, LU, P, info = gpu_getrf(A)
+    amb = np.nonzero(gaps <= 64 * 200 * EPS64)[0]
+    k0 = int(amb[0]) if len(amb) else 200
+    assert np.array_equal(P[:k0], Po[:k0])
+    if mattype in (0, 3, 4):
+        assert lu_residual(A, LU, P) <= 8 * 200 * EPS64
+    if mattype == 0:
+        assert info == 0
+    assert np.isfinite(LU).all()
+
+
+def test_getrf_zero_column_rule(orc, dev):
+    """A column whose candidates underflow is zeroed from the diagonal down
+    and interchanged with itself (src/TMV_LUDecompose.cpp:162-169)."""
+    n = 70
+    A = rng.uniform_matrix(SEED + 41, n, n)
+    A[:, 5] = 1e-320          # subnormal: below DBL_MIN
+    LUo, Po = orc.getrf(A)
+    _, LU, P, info = gpu_getrf(A)
+    assert np.array_equal(P, Po)
+    assert P[5] == 5 and info == 6
+    assert np.all(LU[5:, 5] == 0) and np.all(LUo[5:, 5] == 0)
+
+
+def test_det_fixtures_gpu(orc, dev):
+    mats = F.det_matrices()
+    for name, (true, k) in F.KNOWN["dets"].items():
+        m = mats[name]
+        _, LU, P, _ = gpu_getrf(m)
+        d = orc.det_from_lu(LU, P)
+        norm_m = mats["m11"] if name == "m12" else m
+        assert abs(d - true) <= k * float(np.sum(norm_m ** 2)) * 10 * EPS64, name
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (4, 1), (33, 3), (100, 1), (257, 16), (512, 1),
+                                    (1000, 16)])
+@pytest.mark.parametrize("trans", [False, True])
+def test_getrs(orc, dev, n, nrhs, trans):
+    print(f"getrs n={n} nrhs={nrhs} trans={trans}", flush=True)
+    A = rng.uniform_matrix(SEED + 500 + n, n, n)
+    B = rng.uniform_matrix(SEED + 600 + n, n, nrhs)
+    Ad, LU, P, info = gpu_getrf(A)
+    pivd = torch.from_numpy(P).cuda()
+    Bd = to_dev(B)
+    tmv_amd.getrs(Ad, pivd, Bd, trans=trans)
+    X = to_host(Bd)
+    At = A.T if trans else A
+    for j in range(nrhs):
+        assert backward_error(At, X[:, j], B[:, j]) <= 8 * n * EPS64
+    Xo = orc.getrs(LU, P, B, trans=trans)
+    np.testing.assert_allclose(X, Xo, rtol=0, atol=1e4 * n * EPS64 * np.abs(Xo).max())
+
+
+def test_division_example_gpu(dev):
+    A, b = F.division_example()
+    Ad, LU, P, _ = gpu_getrf(A)
+    bd = to_dev(b)
+    tmv_amd.getrs(Ad, torch.from_numpy(P).cuda(), bd)
+    np.testing.assert_allclose(to_host(bd), F.KNOWN["division_example"]["x"], rtol=5e-6)
+
+
+def test_cfg1_solve_matches_tmv(dev, ref_vectors):
+    """x = b/A for A + 512 I (the CPU-runnable configuration): pivots equal
+    to TMV's and x within 8 n eps backward error and close to TMV's x."""
+    n = 512
+    A = rng.uniform_matrix(SEED + 0, n, n) + n * np.eye(n)
+    b = rng.uniform(SEED + 2, n, -1.0, 1.0)
+    Ad, LU, P, _ = gpu_getrf(A)
+    assert np.array_equal(P, ref_vectors["cfg1_lu_P"])
+    bd = to_dev(b)
+    tmv_amd.getrs(Ad, torch.from_numpy(P).cuda(), bd)
+    x = to_host(bd)
+    assert backward_error(A, x, b) <= 8 * n * EPS64
+    np.testing.assert_allclose(x, ref_vectors["cfg1_x"], rtol=0, atol=1e-14)
+
+
+def test_singular_solve_reports(dev):
+    A = F.decomp_matrix(3, N=20)
+    Ad, LU, P, info = gpu_getrf(A)
+    assert info > 0
+    bd = to_dev(np.ones(20))
+    with pytest.raises(tmv_amd.SingularError):
+        tmv_amd.getrs(Ad, torch.from_numpy(P).cuda(), bd)
+
+
+@pytest.mark.parametrize("n", [64, 300, 1000])
+def test_sgetrf(orc, dev, n):
+    A = rng.uniform_matrix(SEED + 700 + n, n, n).astype(np.float32)
+    LUo, Po, gaps = orc.getrf(A, dtype=np.float32, gaps=True)
+    _, LU, P, info = gpu_getrf(A, torch.float32)
+    ok = gaps > 64 * n * EPS32
+    k0 = n if ok.all() else int(np.argmin(ok))
+    assert np.array_equal(P[:k0], Po[:k0])
+    if n <= 300:
+        assert lu_residual(A, LU.astype(np.float32), P) <= 8 * n * EPS32
+
+
+def test_getrf_8192_backward_error(dev):
+    """Full size n = 8192 (uniform [-1,1], no shift, 16 RHS): solve
+    backward error <= 8 n eps, evaluated on the host in float64."""
+    n, nrhs = 8192, 16
+    A = torch.empty(n * n, dtype=torch.float64, device="cuda")
+    tmv_amd.fill_uniform(A, SEED + 50)
+    A = A.view(n, n).t()
+    B = torch.empty(nrhs * n, dtype=torch.float64, device="cuda")
+    tmv_amd.fill_normal(B, SEED + 51)
+    B = B.view(nrhs, n).t()
+    A0 = to_host(A)
+    B0 = to_host(B)
+    piv, info = tmv_amd.getrf(A)
+    assert info == 0
+    tmv_amd.getrs(A, piv, B)
+    X = to_host(B)
+    R = B0 - A0 @ X
+    nA = np.abs(A0).sum(axis=1).max()
+    be = np.abs(R).max(axis=0) / (nA * np.abs(X).max(axis=0) + np.abs(B0).max(axis=0))
+    assert np.all(be <= 8 * n * EPS64), be.max()

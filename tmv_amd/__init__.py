@@ -1,0 +1,246 @@
+"""tmv_amd -- MI355X (gfx950) dense GEMM + LU behind TMV's interface.
+
+The product is ``tmv_amd/libtmvx.so`` (HIP kernels + the C ABI declared in
+``include/tmvx.h``).  This module is the Python host mirror of that ABI:
+thin ctypes bindings taking torch device tensors (torch is used only for
+device memory and streams).  There is no CPU fallback: every call goes to
+the HIP library and fails loudly when it is missing.
+
+Matrix convention: a column-major m x n matrix is a torch tensor of shape
+(m, n) with stride (1, ld), e.g. ``colmajor(t)`` or ``torch.empty(n, m).t()``;
+a row-major one has stride (ld, 1).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtmvx.so")
+
+OK = 0
+ERR_ARG = -1
+ERR_ALLOC = -2
+ERR_HIP = -3
+ERR_SINGULAR = -4
+ERR_NOT_CONVERGED = -5
+ERR_TIMEOUT = -6
+ERR_UNSUPPORTED = -7
+
+COLMAJOR = 0
+ROWMAJOR = 1
+
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_vp = ctypes.c_void_p
+_d = ctypes.c_double
+_f = ctypes.c_float
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "tmvx_status_str": (ctypes.c_char_p, [_int]),
+    "tmvx_version": (ctypes.c_char_p, []),
+    "tmvx_gemm_f64": (_int, [_int, _int, _i64, _i64, _i64, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64, _vp]),
+    "tmvx_gemm_f32": (_int, [_int, _int, _i64, _i64, _i64, _f, _vp, _i64, _vp, _i64, _f, _vp, _i64, _vp]),
+    "tmvx_getrf_f64": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
+    "tmvx_getrf_f32": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
+    "tmvx_getrs_f64": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _int, _vp]),
+    "tmvx_getrs_f32": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _int, _vp]),
+    "tmvx_trsm_f64": (_int, [_int, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "tmvx_trsm_f32": (_int, [_int, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "tmvx_laswp_f64": (_int, [_int, _i64, _vp, _i64, _i64, _i64, _vp, _vp]),
+    "tmvx_laswp_f32": (_int, [_int, _i64, _vp, _i64, _i64, _i64, _vp, _vp]),
+    "tmvx_fill_uniform_f64": (_int, [ctypes.c_uint64, ctypes.c_uint64, _i64, _d, _d, _vp, _vp]),
+    "tmvx_fill_normal_f64": (_int, [ctypes.c_uint64, ctypes.c_uint64, _i64, _vp, _vp]),
+}
+
+_lib = None
+
+
+class TmvxError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        self.status = status
+        super().__init__(f"{where}: {status_str(status)} ({status})")
+
+
+class SingularError(TmvxError):
+    """A zero diagonal met a non-zero right-hand side (TMV's Singular)."""
+
+
+def lib() -> ctypes.CDLL:
+    """The HIP library.  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m tmv_amd.build`"
+                " (there is no CPU fallback)")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def declared_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def status_str(status: int) -> str:
+    return lib().tmvx_status_str(status).decode()
+
+
+def _check(rc: int, where: str) -> None:
+    if rc == ERR_SINGULAR:
+        raise SingularError(rc, where)
+    if rc != OK:
+        raise TmvxError(rc, where)
+
+
+# ---------------------------------------------------------------------------
+# torch plumbing
+# ---------------------------------------------------------------------------
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream(stream=None) -> int:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def colmajor(t):
+    """Column-major copy of a 2-D tensor (stride (1, m))."""
+    return t.t().contiguous().t()
+
+
+def empty_colmajor(m, n, dtype, device="cuda"):
+    torch = _torch()
+    return torch.empty((n, m), dtype=dtype, device=device).t()
+
+
+def _layout(t, name: str):
+    """(layout, ld) of a 2-D tensor that is column- or row-major."""
+    if t.dim() != 2:
+        raise ValueError(f"{name} must be 2-D")
+    m, n = t.shape
+    s0, s1 = t.stride()
+    if s0 == 1 and (n <= 1 or s1 >= max(1, m)):
+        return COLMAJOR, max(s1, m, 1)
+    if s1 == 1 and (m <= 1 or s0 >= max(1, n)):
+        return ROWMAJOR, max(s0, n, 1)
+    raise ValueError(f"{name} must be column- or row-major with unit stride")
+
+
+def _colmajor_ld(t, name: str) -> int:
+    lay, ld = _layout(t, name)
+    if lay != COLMAJOR:
+        raise ValueError(f"{name} must be column-major")
+    return ld
+
+
+def _suffix(dtype) -> str:
+    torch = _torch()
+    if dtype == torch.float64:
+        return "f64"
+    if dtype == torch.float32:
+        return "f32"
+    raise TypeError(f"unsupported dtype {dtype}")
+
+
+# ---------------------------------------------------------------------------
+# The operations (mirror TMV: MultMM, LU_Decompose, LU_LDivEq / LU_RDivEq)
+# ---------------------------------------------------------------------------
+
+def gemm(A, B, C=None, alpha=1.0, beta=0.0, stream=None):
+    """C := alpha*A*B + beta*C (TMV MultMM<add> with beta = add).  A and B
+    may be column- or row-major; C (allocated column-major when None) must
+    be column-major.  Returns C."""
+    torch = _torch()
+    m, k = A.shape
+    k2, n = B.shape
+    if k != k2:
+        raise ValueError("inner dimensions differ")
+    la, lda = _layout(A, "A")
+    lb, ldb = _layout(B, "B")
+    if C is None:
+        C = empty_colmajor(m, n, A.dtype, A.device)
+        beta = 0.0
+    ldc = _colmajor_ld(C, "C")
+    suf = _suffix(A.dtype)
+    if B.dtype != A.dtype or C.dtype != A.dtype:
+        raise TypeError("A, B and C must share a dtype")
+    fn = getattr(lib(), "tmvx_gemm_" + suf)
+    _check(fn(la, lb, m, n, k, alpha, A.data_ptr(), lda, B.data_ptr(), ldb, beta,
+              C.data_ptr(), ldc, _stream(stream)), "gemm")
+    return C
+
+
+def getrf(A, stream=None):
+    """In-place PA = LU of a square column-major tensor (TMV LU_Decompose).
+    Returns (piv, info): piv an int64 device tensor of 0-based interchanges,
+    info the 1-based index of the first zero pivot (0 if none)."""
+    torch = _torch()
+    n = A.shape[0]
+    if A.shape[1] != n:
+        raise ValueError("square matrices only")
+    lda = _colmajor_ld(A, "A")
+    piv = torch.empty(n, dtype=torch.int64, device=A.device)
+    info = _i64(0)
+    fn = getattr(lib(), "tmvx_getrf_" + _suffix(A.dtype))
+    _check(fn(n, A.data_ptr(), lda, piv.data_ptr(), ctypes.byref(info), _stream(stream)),
+           "getrf")
+    return piv, int(info.value)
+
+
+def getrs(LU, piv, B, trans=False, check_singular=True, stream=None):
+    """In-place solve with the factors: B := A^-1 B (trans: A^-T B).
+    B is column-major n x nrhs (a 1-D tensor is treated as n x 1)."""
+    n = LU.shape[0]
+    lda = _colmajor_ld(LU, "LU")
+    Bm = B.view(n, 1) if B.dim() == 1 else B
+    ldb = _colmajor_ld(Bm, "B") if Bm.shape[1] > 1 else max(n, 1)
+    fn = getattr(lib(), "tmvx_getrs_" + _suffix(LU.dtype))
+    _check(fn(int(bool(trans)), n, Bm.shape[1], LU.data_ptr(), lda, piv.data_ptr(),
+              Bm.data_ptr(), ldb, int(bool(check_singular)), _stream(stream)), "getrs")
+    return B
+
+
+def trsm(A, B, upper, trans=False, unit=False, stream=None):
+    n = A.shape[0]
+    lda = _colmajor_ld(A, "A")
+    Bm = B.view(n, 1) if B.dim() == 1 else B
+    ldb = _colmajor_ld(Bm, "B") if Bm.shape[1] > 1 else max(n, 1)
+    fn = getattr(lib(), "tmvx_trsm_" + _suffix(A.dtype))
+    _check(fn(int(bool(upper)), int(bool(trans)), int(bool(unit)), n, Bm.shape[1],
+              A.data_ptr(), lda, Bm.data_ptr(), ldb, _stream(stream)), "trsm")
+    return B
+
+
+def laswp(A, piv, k1=0, k2=None, forward=True, stream=None):
+    m, ncols = A.shape if A.dim() == 2 else (A.shape[0], 1)
+    k2 = piv.numel() if k2 is None else k2
+    lda = _colmajor_ld(A, "A") if A.dim() == 2 and ncols > 1 else max(m, 1)
+    fn = getattr(lib(), "tmvx_laswp_" + _suffix(A.dtype))
+    _check(fn(int(bool(forward)), ncols, A.data_ptr(), lda, k1, k2, piv.data_ptr(),
+              _stream(stream)), "laswp")
+    return A
+
+
+def fill_uniform(out, seed, lo=-1.0, hi=1.0, offset=0, stream=None):
+    """Fill a contiguous float64 device buffer (flat counter order) with the
+    documented SplitMix64 uniform stream (see tmv_amd.rng)."""
+    _check(lib().tmvx_fill_uniform_f64(seed, offset, out.numel(), lo, hi, out.data_ptr(),
+                                       _stream(stream)), "fill_uniform")
+    return out
+
+
+def fill_normal(out, seed, offset=0, stream=None):
+    _check(lib().tmvx_fill_normal_f64(seed, offset, out.numel(), out.data_ptr(),
+                                      _stream(stream)), "fill_normal")
+    return out
