@@ -12,21 +12,25 @@ namespace tmvx {
 
 typedef int64_t i64;
 
-// MFMA 16x16x4 wrappers.  Operand layout (one value per lane, lane l):
-//   a = A[i = l%16][k = l/16],  b = B[k = l/16][j = l%16],
-//   acc[r] = C[i = 4*(l/16) + r][j = l%16],  r = 0..3.
+// MFMA 16x16x4 wrappers.  Operand layout (one value per lane, lane l,
+// g = l/16):  a = A[i = l%16][k = g],  b = B[k = g][j = l%16].
+// Result acc[r] = C[row(g, r)][j = l%16] with (measured on gfx950)
+//   f64 16x16x4: row = g + 4r   (rows interleaved across the 4 registers)
+//   f32 16x16x4: row = 4g + r   (4 consecutive rows per lane)
 template <typename T> struct Mfma;
 
 template <> struct Mfma<double> {
     typedef double v4 __attribute__((ext_vector_type(4)));
     static __device__ __forceinline__ v4 mac(double a, double b, v4 c)
     { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int g, int r) { return g + 4 * r; }
 };
 
 template <> struct Mfma<float> {
     typedef float v4 __attribute__((ext_vector_type(4)));
     static __device__ __forceinline__ v4 mac(float a, float b, v4 c)
     { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
 };
 
 // 16-byte vector of T.

@@ -188,20 +188,21 @@ gemm_kernel(i64 M, i64 N, i64 K, T alpha, const T* __restrict__ A, i64 lda,
         }
     }
 
-    // ---- epilogue: acc[i][j][r] = C[4g + r][r16] of block (i, j) --------
+    // ---- epilogue: acc[i][j][r] = C[row(g, r)][r16] of block (i, j) -----
 #pragma unroll
     for (int i = 0; i < Cfg::FM; ++i) {
 #pragma unroll
         for (int j = 0; j < Cfg::FN; ++j) {
             const i64 col = n0 + wn * Cfg::TN + 16 * j + r16;
-            const i64 row0 = m0 + wm * Cfg::TM + 16 * i + 4 * g;
+            const i64 rowb = m0 + wm * Cfg::TM + 16 * i;
             if (col >= N) continue;
-            T* cp = C + row0 + col * ldc;
+            T* cp = C + col * ldc;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                if (row0 + r < M) {
+                const i64 row = rowb + Mfma<T>::row(g, r);
+                if (row < M) {
                     const T v = alpha * acc[i][j][r];
-                    cp[r] = (beta == T(0)) ? v : v + beta * cp[r];
+                    cp[row] = (beta == T(0)) ? v : v + beta * cp[row];
                 }
             }
         }
