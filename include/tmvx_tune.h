@@ -1,0 +1,36 @@
+/*
+ * tmvx_tune.h -- tuning hooks of the tmvx library (no counterpart in TMV;
+ * TMV picks its GEMM blocking at compile time, src/TMV_MultMM_Block.cpp:
+ * 119-164).  Used by the autotuner and the benchmark to time every GEMM
+ * tile configuration; tmvx_gemm_* picks the default configuration itself.
+ */
+#ifndef TMVX_TUNE_H
+#define TMVX_TUNE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Number of GEMM tile configurations for an element size (8 or 4 bytes). */
+int tmvx_gemm_config_count(int elem_bytes);
+
+/* Text description of configuration `cfg` ("BMxBNxBK waves WMxWN ..."). */
+const char* tmvx_gemm_config_name(int elem_bytes, int cfg);
+
+/* tmvx_gemm_f64 / _f32 with an explicit tile configuration. */
+int tmvx_gemm_cfg_f64(int cfg, int layout_a, int layout_b, int64_t m, int64_t n,
+                      int64_t k, double alpha, const double* A, int64_t lda,
+                      const double* B, int64_t ldb, double beta, double* C,
+                      int64_t ldc, void* stream);
+int tmvx_gemm_cfg_f32(int cfg, int layout_a, int layout_b, int64_t m, int64_t n,
+                      int64_t k, float alpha, const float* A, int64_t lda,
+                      const float* B, int64_t ldb, float beta, float* C,
+                      int64_t ldc, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TMVX_TUNE_H */

@@ -31,17 +31,41 @@ FLAGS = [
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, "-I", CSRC, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
     return obj
 
 
+def _gen_gemm_instances() -> list[str]:
+    """One translation unit per GEMM tile configuration listed in
+    csrc/tmvx_gemm_cfgs.h, so the instantiations compile in parallel."""
+    import re
+    gen = os.path.join(BUILD, "gen")
+    os.makedirs(gen, exist_ok=True)
+    table = open(os.path.join(CSRC, "tmvx_gemm_cfgs.h")).read()
+    out = []
+    for dt, macro in (("double", "TMVX_F64_CFGS"), ("float", "TMVX_F32_CFGS")):
+        body = table.split("#define " + macro, 1)[1].split("\n\n", 1)[0]
+        for m in re.finditer(r"X\((\d+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+)\)", body):
+            cid, bm, bn, bk, wm, wn, mw = m.groups()
+            cfg = f"GemmCfg<{dt}, {bm}, {bn}, {bk}, {wm}, {wn}, {mw}>"
+            src = (f'#include "tmvx_gemm_kernel.h"\nnamespace tmvx {{\n'
+                   f"template int gemm_launch_cfg<{cfg}, {dt}>(int, int, i64, i64, i64, {dt}, "
+                   f"const {dt}*, i64, const {dt}*, i64, {dt}, {dt}*, i64, hipStream_t);\n}}\n")
+            path = os.path.join(gen, f"gemm_{dt}_{cid}.hip")
+            if not os.path.exists(path) or open(path).read() != src:
+                with open(path, "w") as f:
+                    f.write(src)
+            out.append(path)
+    return out
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip"))) + _gen_gemm_instances()
+    with cf.ThreadPoolExecutor(max_workers=min(os.cpu_count() or 8, 16, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]

@@ -115,13 +115,22 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
 #pragma unroll
     for (int c = 0; c < NB; ++c) row[c] = (own && c < jb) ? gp[(i64)c * lda] : T(0);
 
-    // Candidate for column 0.
-    T cv = (own) ? fabs(row[0]) : T(-1);
-    bool alive = true;
+    // Rows are kept rotated: before column k is processed, row[c] holds
+    // panel column (k + c) mod NB, so the current column is always row[0]
+    // and every register index is static.  Each step rotates by one; the
+    // rows (and the published pivot / diagonal rows) share the rotation.
+    auto rotate = [&]() {
+        const T first = row[0];
+#pragma unroll
+        for (int c = 0; c < NB - 1; ++c) row[c] = row[c + 1];
+        row[NB - 1] = first;
+    };
 
-    static_for<0, NB>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        if (k >= jb || !alive) return;           // uniform across the grid
+    T cv = own ? fabs(row[0]) : T(-1);    // candidate for column 0
+    bool alive = true;
+    int nrot = 0;
+
+    for (int k = 0; k < jb && alive; ++k) {
         const int par = k & 1;
         // -- local argmax over rows >= k --------------------------------
         T v = (own && r >= k) ? cv : T(-1);
@@ -151,7 +160,7 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
         }
         if (!grid_barrier(wk.ctr, base + (unsigned)(G * (k + 1)), wk.err)) {
             alive = false;
-            return;
+            break;
         }
         // -- global winner (identical in every workgroup) -----------------
         if (wave == 0) {
@@ -178,7 +187,7 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
         __syncthreads();
         const int p = s_p;
         if (s_zero) {
-            if (own && r >= k) row[k] = T(0);
+            if (own && r >= k) row[0] = T(0);
             if (w == 0 && t == 0) {
                 piv[j0 + k] = j0 + k;
                 atomicMin(reinterpret_cast<unsigned long long*>(wk.first_zero),
@@ -199,17 +208,23 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
                 for (int c = 0; c < NB; ++c) row[c] = s_prow[c];
             }
             if (own && r > k) {
-                const T inv = T(1) / s_prow[k];
-                const T l = row[k] * inv;
-                row[k] = l;
+                const T inv = T(1) / s_prow[0];
+                const T l = row[0] * inv;
+                row[0] = l;
+                // rotated columns 1 .. NB-1-k are the panel columns k+1 .. NB-1
 #pragma unroll
-                for (int c = k + 1; c < NB; ++c) row[c] -= l * s_prow[c];
+                for (int c = 1; c < NB; ++c)
+                    if (c < NB - k) row[c] -= l * s_prow[c];
             }
             if (w == 0 && t == 0) piv[j0 + k] = j0 + p;
         }
-        if constexpr (k + 1 < NB) cv = own ? fabs(row[k + 1]) : T(-1);
+        rotate();
+        ++nrot;
+        cv = own ? fabs(row[0]) : T(-1);
         __syncthreads();
-    });
+    }
+    // Restore column order (jb < NB leaves the rows part-way round).
+    for (int q = nrot % NB; q != 0 && q < NB; ++q) rotate();
 
 #pragma unroll
     for (int c = 0; c < NB; ++c)

@@ -1,0 +1,100 @@
+"""GEMM tile-configuration sweep (the tuning tool).
+
+    python -m tmv_amd.tune [--n 8192] [--dtype f64|f32|both] [--reps 3]
+
+Times every configuration of the tile table (include/tmvx_tune.h) on each
+operand layout with HIP events and prints TFLOP/s; --json writes the table.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+
+import torch
+
+import tmv_amd
+
+_i64, _int, _vp = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p
+
+
+def _bind():
+    L = tmv_amd.lib()
+    L.tmvx_gemm_config_count.restype = _int
+    L.tmvx_gemm_config_count.argtypes = [_int]
+    L.tmvx_gemm_config_name.restype = ctypes.c_char_p
+    L.tmvx_gemm_config_name.argtypes = [_int, _int]
+    for suf, ct in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
+        f = getattr(L, "tmvx_gemm_cfg_" + suf)
+        f.restype = _int
+        f.argtypes = [_int, _int, _int, _i64, _i64, _i64, ct, _vp, _i64, _vp, _i64, ct, _vp,
+                      _i64, _vp]
+    return L
+
+
+def sweep(n=8192, dtypes=("f64", "f32"), reps=3, layouts=("CC", "CR", "RC", "RR"),
+          m=None, k=None):
+    L = _bind()
+    m = m or n
+    k = k or n
+    out = []
+    s = torch.cuda.current_stream().cuda_stream
+    for suf in dtypes:
+        dt = torch.float64 if suf == "f64" else torch.float32
+        eb = 8 if suf == "f64" else 4
+        A = torch.empty(m * k, dtype=torch.float64, device="cuda")
+        B = torch.empty(k * n, dtype=torch.float64, device="cuda")
+        tmv_amd.fill_uniform(A, 1)
+        tmv_amd.fill_uniform(B, 2)
+        A, B = A.to(dt), B.to(dt)
+        C = torch.empty(m * n, dtype=dt, device="cuda")
+        fn = getattr(L, "tmvx_gemm_cfg_" + suf)
+        for cfg in range(L.tmvx_gemm_config_count(eb)):
+            name = L.tmvx_gemm_config_name(eb, cfg).decode()
+            for lay in layouts:
+                la = 0 if lay[0] == "C" else 1
+                lb = 0 if lay[1] == "C" else 1
+                lda = m if la == 0 else k
+                ldb = k if lb == 0 else n
+                args = (cfg, la, lb, m, n, k, 1.0, A.data_ptr(), lda, B.data_ptr(), ldb, 0.0,
+                        C.data_ptr(), m, s)
+                rc = fn(*args)
+                if rc != 0:
+                    print(f"{suf} cfg {cfg} {lay}: rc={rc}")
+                    continue
+                torch.cuda.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    fn(*args)
+                e1.record()
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                tf = 2.0 * m * n * k / ms / 1e9
+                out.append({"dtype": suf, "cfg": cfg, "name": name, "layout": lay,
+                            "m": m, "n": n, "k": k, "ms": ms, "tflops": tf})
+                print(f"{suf} cfg {cfg} [{name}] {lay} {m}x{n}x{k}: {ms:8.3f} ms {tf:7.2f} TFLOP/s",
+                      flush=True)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--m", type=int, default=None)
+    ap.add_argument("--k", type=int, default=None)
+    ap.add_argument("--dtype", default="both")
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--layouts", default="CC,CR,RC,RR")
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args()
+    dts = ("f64", "f32") if a.dtype == "both" else (a.dtype,)
+    res = sweep(a.n, dts, a.reps, tuple(a.layouts.split(",")), a.m, a.k)
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump(res, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
