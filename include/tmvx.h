@@ -128,6 +128,37 @@ int tmvx_laswp_f32(int forward, int64_t ncols, float* A, int64_t lda,
                    int64_t k1, int64_t k2, const int64_t* piv, void* stream);
 
 /*
+ * Mixed-precision solve with iterative refinement (no counterpart in TMV,
+ * whose solves are all in the matrix's own precision; the seam it extends
+ * is x = b/A with divideUsing(LU), src/TMV_LUDiv.cpp:185-226).
+ * A (n x n, FP64, not modified) is factorised in low precision on MFMA
+ * (PA = LU, FP32 arithmetic), then for each of the nrhs columns
+ *   r = b - A x  (FP64, on the GPU),  solve LU d = P r / s in FP32 with
+ *   s = max|r|,  x += s d  (FP64)
+ * until the normwise backward error ||r||_inf / (||A||_inf ||x||_inf +
+ * ||b||_inf) of every column is <= tol, or max_iter refinement steps.
+ * X (n x nrhs, FP64) receives the solution.  Synchronises.  Returns
+ * TMVX_ERR_NOT_CONVERGED (with X holding the best iterate) when the target
+ * is not met.
+ */
+enum tmvx_lowprec { TMVX_LOWPREC_F32 = 0 };
+
+typedef struct tmvx_ir_info {
+    int iterations;            /* refinement steps taken                  */
+    int converged;             /* 1 when every column met tol             */
+    int factor_info;           /* first zero pivot of the low-precision LU */
+    int reserved;
+    double backward_error;     /* worst column, final iterate             */
+    double factor_ms;          /* device time of the factorisation        */
+    double refine_ms;          /* device time of all refinement steps     */
+} tmvx_ir_info;
+
+int tmvx_gesv_mixed_f64(int lowprec, int64_t n, int64_t nrhs, const double* A,
+                        int64_t lda, const double* B, int64_t ldb, double* X,
+                        int64_t ldx, double tol, int max_iter,
+                        tmvx_ir_info* info, void* stream);
+
+/*
  * Synthetic inputs: out[i] = lo + (hi-lo) * u(seed, offset + i), u in [0,1)
  * from the counter-based SplitMix64 stream documented in tmv_amd/rng.py
  * (bit-identical to it).  normal: Box-Muller over pairs of that stream.

@@ -52,6 +52,12 @@ _SIGS = {
     "tmvx_laswp_f32": (_int, [_int, _i64, _vp, _i64, _i64, _i64, _vp, _vp]),
     "tmvx_fill_uniform_f64": (_int, [ctypes.c_uint64, ctypes.c_uint64, _i64, _d, _d, _vp, _vp]),
     "tmvx_fill_normal_f64": (_int, [ctypes.c_uint64, ctypes.c_uint64, _i64, _vp, _vp]),
+    "tmvx_gesv_mixed_f64": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _d, _int, _vp, _vp]),
+    # tuning hooks (include/tmvx_tune.h)
+    "tmvx_gemm_config_count": (_int, [_int]),
+    "tmvx_gemm_config_name": (ctypes.c_char_p, [_int, _int]),
+    "tmvx_gemm_cfg_f64": (_int, [_int, _int, _int, _i64, _i64, _i64, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64, _vp]),
+    "tmvx_gemm_cfg_f32": (_int, [_int, _int, _int, _i64, _i64, _i64, _f, _vp, _i64, _vp, _i64, _f, _vp, _i64, _vp]),
 }
 
 _lib = None
@@ -230,6 +236,46 @@ def laswp(A, piv, k1=0, k2=None, forward=True, stream=None):
     _check(fn(int(bool(forward)), ncols, A.data_ptr(), lda, k1, k2, piv.data_ptr(),
               _stream(stream)), "laswp")
     return A
+
+
+class IrInfo(ctypes.Structure):
+    """struct tmvx_ir_info (include/tmvx.h)."""
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("factor_info", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("backward_error", ctypes.c_double), ("factor_ms", ctypes.c_double),
+                ("refine_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+LOWPREC_F32 = 0
+
+
+def gesv_mixed(A, B, X=None, tol=1e-12, max_iter=30, lowprec=LOWPREC_F32, check=True,
+               stream=None):
+    """Mixed-precision solve with iterative refinement: A (FP64, column-major,
+    not modified) factorised in FP32 on MFMA, residual and update in FP64.
+    Returns (X, info dict).  Raises on failure when check is set."""
+    torch = _torch()
+    n = A.shape[0]
+    lda = _colmajor_ld(A, "A")
+    Bm = B.view(n, 1) if B.dim() == 1 else B
+    ldb = _colmajor_ld(Bm, "B") if Bm.shape[1] > 1 else max(n, 1)
+    nrhs = Bm.shape[1]
+    if X is None:
+        X = empty_colmajor(n, nrhs, torch.float64, A.device)
+    Xm = X.view(n, 1) if X.dim() == 1 else X
+    ldx = _colmajor_ld(Xm, "X") if Xm.shape[1] > 1 else max(n, 1)
+    info = IrInfo()
+    rc = lib().tmvx_gesv_mixed_f64(lowprec, n, nrhs, A.data_ptr(), lda, Bm.data_ptr(), ldb,
+                                   Xm.data_ptr(), ldx, tol, max_iter, ctypes.byref(info),
+                                   _stream(stream))
+    if check:
+        _check(rc, "gesv_mixed")
+    d = info.as_dict()
+    d["status"] = rc
+    return (X if B.dim() > 1 else Xm.view(n)), d
 
 
 def fill_uniform(out, seed, lo=-1.0, hi=1.0, offset=0, stream=None):

@@ -1,0 +1,43 @@
+// Internal interfaces of the LU factorisation and triangular solves, shared
+// by the plain, mixed-precision and adjoint drivers.
+#pragma once
+#include "tmvx_common.h"
+
+namespace tmvx {
+
+// In-place PA = LU (pivots 0-based interchanges).  When perm != nullptr it
+// also receives the composed row permutation: row i of PA is row perm[i] of
+// A.  Synchronises; *info = 1-based first zero pivot or 0.
+template <typename T>
+int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s);
+
+// Interchange list -> composed permutation (row i of PA = row perm[i] of A).
+int piv_to_perm(i64 n, const i64* piv, i64* perm, hipStream_t s);
+
+// op(A) X = B for triangular A (trsm semantics of tmvx_trsm_*).  When
+// in_perm != nullptr the right-hand side is first gathered, B'[i] =
+// B[in_perm[i]]; when out_perm != nullptr the solution is scattered,
+// X row i -> B row out_perm[i].  d_singular (device int) is or-ed with 1 if
+// a zero diagonal meets a non-zero entry.
+template <typename T>
+int trsm_impl(int upper, int trans, int unit, i64 n, i64 nrhs, const T* A, i64 lda,
+              T* B, i64 ldb, const i64* in_perm, const i64* out_perm, int* d_singular,
+              hipStream_t s);
+
+// Single-launch-per-4-RHS blocked triangular solve (tmvx_trsv.hip):
+// op(A) X = B with B rows gathered through in_perm and X rows scattered
+// through out_perm.  X may alias B only when both permutations are null.
+template <typename T>
+int trsv_multi(int upper, int trans, int unit, i64 n, i64 nrhs, const T* A, i64 lda,
+               const T* B, i64 ldb, const i64* in_perm, T* X, i64 ldx, const i64* out_perm,
+               int* d_singular, hipStream_t s);
+
+// Solve with the factors given the composed permutation (see getrf_impl).
+template <typename T>
+int getrs_perm(int trans, i64 n, i64 nrhs, const T* LU, i64 lda, const i64* perm, T* B,
+               i64 ldb, int* d_singular, hipStream_t s);
+
+// Fill a device buffer with zeros (a kernel, so it may be graph-captured).
+int zero_bytes(void* p, size_t bytes, hipStream_t s);
+
+} // namespace tmvx

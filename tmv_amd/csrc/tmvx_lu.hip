@@ -1,28 +1,35 @@
-// Blocked right-looking LU with partial pivoting, triangular solves and
-// row interchanges for gfx950.
+// Blocked right-looking LU with partial pivoting for gfx950.
 //
 // Counterpart of TMV's LU_Decompose / RecursiveLUDecompose
-// (src/TMV_LUDecompose.cpp:217-407, 514-559), LU_LDivEq / LU_RDivEq
-// (src/TMV_LUDiv.cpp:185-226, 369-410), NonBlasTriLDivEq (src/TMV_TriDiv_
-// M.cpp:190-266), FindMaxAbsElement (src/TMV_Vector.cpp:473-504) and
-// permuteRows (src/TMV_Matrix.cpp:865-930).
+// (src/TMV_LUDecompose.cpp:217-407, 514-559), FindMaxAbsElement
+// (src/TMV_Vector.cpp:473-504) and permuteRows (src/TMV_Matrix.cpp:865-930).
+// TMV recurses on column halves down to 1-2 column base cases; here the
+// matrix is processed in panels of NB = 64 columns.  On tie-free inputs both
+// choose the same pivots (GEPP's pivot sequence is fixed by the arithmetic
+// of exact elimination); factors agree to rounding.
 //
 // Per panel of NB columns starting at j0:
 //   1. panel_kernel: G workgroups each hold a contiguous slab of the panel's
-//      rows in registers (one row per thread).  Per column: local argmax
-//      (largest |a|, lowest row on ties, as TMV's strict '>'), candidates
-//      published to a slot, ONE grid barrier, every workgroup reduces the G
-//      candidates identically, swaps, scales by the reciprocal of the pivot
-//      and applies the rank-1 update to its rows, fusing the next column's
-//      argmax.  Slots are double-buffered by column parity so one barrier
-//      per column suffices.  The underflow rule of TMV (|pivot| < min:
-//      zero the column from the diagonal down, interchange with itself)
-//      is kept.  Every wait is bounded; an expired wait raises an error
-//      flag and the kernel drains.
-//   2. laswp_kernel: the panel's interchanges on the columns outside it.
-//   3. trsm_u12_kernel: U12 = L11^-1 A12 with L11 in LDS.
+//      rows in registers (one row per thread, kept rotated so the current
+//      column is always register 0).  Per column: local argmax (largest
+//      |a|, lowest row on ties, as TMV's strict '>'); each workgroup
+//      publishes its candidate row, value and index to its slot as tagged
+//      64-bit words (32 payload bits + the global column number), and every
+//      workgroup polls all slots until each word carries this column's tag
+//      -- the grid synchronisation and the data exchange are one round
+//      trip.  Every workgroup then reduces the G candidates identically,
+//      swaps, scales by the reciprocal of the pivot and updates its rows.
+//      Slots are double-buffered by column parity.  TMV's
+//      underflow rule (largest |candidate| < min: zero the column from the
+//      diagonal down, interchange with itself) is kept.  Every wait is
+//      bounded; an expired wait raises an error flag and the kernel drains.
+//   2. laswp_panel_kernel: the panel's net row permutation (at most 2*NB
+//      rows) computed once per workgroup, applied to every column outside
+//      the panel as a one-wave-per-column gather; the composed permutation
+//      vector is updated the same way.
+//   3. trsm_u12_kernel: U12 = L11^-1 A12 (lane i owns row i of L11).
 //   4. gemm_launch: A22 -= A21 * U12 on MFMA.
-#include "tmvx_common.h"
+#include "tmvx_lu.h"
 #include <cfloat>
 #include <climits>
 
@@ -30,7 +37,8 @@ namespace tmvx {
 
 constexpr int PANEL_NB = 64;        // panel width (columns)
 constexpr int PANEL_THREADS = 512;  // rows per workgroup (one per thread)
-constexpr unsigned SPIN_LIMIT = 1u << 21;
+constexpr int PANEL_GMAX = 64;      // workgroups per panel (n <= 32768)
+constexpr unsigned SPIN_LIMIT = 1u << 22;
 
 template <typename T> __device__ __forceinline__ T tiny();
 template <> __device__ __forceinline__ double tiny<double>() { return DBL_MIN; }
@@ -41,74 +49,130 @@ template <typename T>
 __device__ __forceinline__ bool better(T v1, int i1, T v0, int i0)
 { return v1 > v0 || (v1 == v0 && i1 < i0); }
 
+// One DPP step of the wave argmax: combine (v, i) with the candidate held by
+// the lane selected by CTRL (row_mask RM).  Rows outside RM keep theirs.
+template <int CTRL, int RM, typename T>
+__device__ __forceinline__ void dpp_argmax_step(T& v, int& i)
+{
+    T v2;
+    if constexpr (sizeof(T) == 8) {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, RM, 0xf, false);
+        const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, RM, 0xf, false);
+        v2 = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    } else {
+        v2 = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                        CTRL, RM, 0xf, false));
+    }
+    const int i2 = __builtin_amdgcn_update_dpp(i, i, CTRL, RM, 0xf, false);
+    if (better(v2, i2, v, i)) { v = v2; i = i2; }
+}
+
+// Wave argmax by DPP (quad swaps, half-row and row mirrors, row broadcasts);
+// the result is returned to every lane from lane 63.
 template <typename T>
 __device__ __forceinline__ void wave_argmax(T& v, int& i)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        T v2 = __shfl_xor(v, off, 64);
-        int i2 = __shfl_xor(i, off, 64);
-        if (better(v2, i2, v, i)) { v = v2; i = i2; }
-    }
+    dpp_argmax_step<0xB1, 0xf>(v, i);    // quad_perm [1,0,3,2]
+    dpp_argmax_step<0x4E, 0xf>(v, i);    // quad_perm [2,3,0,1]
+    dpp_argmax_step<0x141, 0xf>(v, i);   // row_half_mirror
+    dpp_argmax_step<0x140, 0xf>(v, i);   // row_mirror
+    dpp_argmax_step<0x142, 0xa>(v, i);   // row_bcast:15
+    dpp_argmax_step<0x143, 0xc>(v, i);   // row_bcast:31
+    v = readlane(v, 63);
+    i = __builtin_amdgcn_readlane(i, 63);
 }
 
-// Grid barrier over gridDim.x workgroups: arrive on a monotonically
-// increasing counter, wait until it reaches `target`.  Bounded: after
-// SPIN_LIMIT polls it sets *err and returns false.
-__device__ bool grid_barrier(unsigned* ctr, unsigned target, int* err)
-{
-    __shared__ int s_ok;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        int ok = 1;
-        unsigned spins = 0;
-        while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (++spins > SPIN_LIMIT ||
-                __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        __threadfence();
-        s_ok = ok;
-    }
-    __syncthreads();
-    return s_ok != 0;
-}
+// Device-coherent (agent scope) relaxed loads/stores for the exchange slots:
+// they are performed at the device coherence point, so a workgroup on any
+// XCD observes them without cache maintenance.
+template <typename T>
+__device__ __forceinline__ void cstore(T* p, T v)
+{ __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename T>
+__device__ __forceinline__ T cload(const T* p)
+{ return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 struct PanelWork {
-    void* slot_rows;   // [2][G][NB] T
-    void* slot_val;    // [2][G] T
-    int* slot_idx;     // [2][G]
-    void* diag_rows;   // [2][NB] T
-    unsigned* ctr;     // barrier counter
-    int* err;          // timeout flag
-    i64* first_zero;   // 1-based first zero pivot (min), n+1 if none
+    unsigned long long* slots;   // [2][G][SLOT] tagged words (see SlotLayout)
+    int* err;                    // timeout flag
+    i64* first_zero;             // 1-based first zero pivot (min), n+1 if none
 };
+
+// Exchange slot of one workgroup for one column parity, in tagged 64-bit
+// words: (column tag << 32) | 32 payload bits.  A double takes two words
+// (low, high half), a float one.  Each word is written and read as one
+// 64-bit atomic, so a reader that sees the current tag in a word also sees
+// that word's payload: no separate flag and no ordering fence is needed.
+template <typename T, int NB> struct SlotLayout {
+    static constexpr int WPV = sizeof(T) / 4;             // words per value
+    static constexpr int ROW = 0;                          // candidate row
+    static constexpr int DIAG = NB * WPV;                  // row k (slab 0 only)
+    static constexpr int VAL = 2 * NB * WPV;               // candidate |value|
+    static constexpr int IDX = VAL + WPV;                  // candidate row index
+    static constexpr int SIZE = IDX + 1;
+};
+
+template <typename T>
+__device__ __forceinline__ unsigned half_bits(T v, int h)
+{
+    if constexpr (sizeof(T) == 8) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+        return h ? (unsigned)(b >> 32) : (unsigned)b;
+    } else {
+        return (unsigned)__float_as_uint(v);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T from_halves(const unsigned* h)
+{
+    if constexpr (sizeof(T) == 8)
+        return __longlong_as_double((long long)(((unsigned long long)h[1] << 32) | h[0]));
+    else
+        return __uint_as_float(h[0]);
+}
+
+// Phase timestamps of the panel loop (shader clock), only in a build with
+// -DTMVX_PANEL_TRACE: workgroup 0, thread 0, first traced panel.
+#ifdef TMVX_PANEL_TRACE
+__device__ unsigned long long* g_panel_trace;
+#define PANEL_TRACE(k, ph)                                                     \
+    do {                                                                       \
+        if (g_panel_trace && w == 0 && t == 0 && j0 == 0)                      \
+            g_panel_trace[(k) * 8 + (ph)] = clock64();                         \
+    } while (0)
+#else
+#define PANEL_TRACE(k, ph) do {} while (0)
+#endif
 
 template <typename T, int NB, int NTH>
 __global__ void __launch_bounds__(NTH)
 panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
-             i64* __restrict__ piv, PanelWork wk, unsigned base)
+             i64* __restrict__ piv, PanelWork wk)
 {
+    typedef SlotLayout<T, NB> SL;
+    constexpr int WPV = SL::WPV;
     const int G = gridDim.x, w = blockIdx.x, t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     constexpr int NW = NTH / 64;
     const i64 mp = n - j0;
     const int r = w * NTH + t;                 // panel-relative row
     const bool own = r < mp;
-    T* slot_rows = static_cast<T*>(wk.slot_rows);
-    T* slot_val = static_cast<T*>(wk.slot_val);
-    T* diag_rows = static_cast<T*>(wk.diag_rows);
 
-    __shared__ T s_prow[NB], s_orow[NB];
+    // Received payload halves: candidate rows of every slab, values,
+    // indices, and row k.
+    __shared__ unsigned s_cand[PANEL_GMAX][NB * WPV];
+    __shared__ unsigned s_diag[NB * WPV];
+    __shared__ unsigned s_cval[PANEL_GMAX][WPV];
+    __shared__ int s_cidx[PANEL_GMAX];
+    __shared__ T s_out[2][NB];                 // rows to publish (cand, diag)
+    __shared__ __attribute__((aligned(16))) T s_prow[NB];   // winning row
+    __shared__ __attribute__((aligned(16))) T s_orow[NB];   // row k before the swap
     __shared__ T s_wv[NW];
     __shared__ int s_wi[NW];
-    __shared__ int s_p, s_win, s_zero;
+    __shared__ T s_bv;
+    __shared__ int s_p, s_win, s_zero, s_fail;
 
     T row[NB];
     T* gp = A + (j0 + r) + j0 * lda;
@@ -118,7 +182,7 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
     // Rows are kept rotated: before column k is processed, row[c] holds
     // panel column (k + c) mod NB, so the current column is always row[0]
     // and every register index is static.  Each step rotates by one; the
-    // rows (and the published pivot / diagonal rows) share the rotation.
+    // published candidate and diagonal rows share the rotation.
     auto rotate = [&]() {
         const T first = row[0];
 #pragma unroll
@@ -126,12 +190,16 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
         row[NB - 1] = first;
     };
 
+    if (t == 0) s_fail = 0;
     T cv = own ? fabs(row[0]) : T(-1);    // candidate for column 0
     bool alive = true;
     int nrot = 0;
 
     for (int k = 0; k < jb && alive; ++k) {
         const int par = k & 1;
+        const unsigned long long tag = (unsigned long long)(j0 + k + 1) << 32;
+        unsigned long long* myslot = wk.slots + ((i64)par * G + w) * SL::SIZE;
+        PANEL_TRACE(k, 0);
         // -- local argmax over rows >= k --------------------------------
         T v = (own && r >= k) ? cv : T(-1);
         int vi = r;
@@ -142,49 +210,92 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
             T bv = s_wv[0]; int bi = s_wi[0];
             for (int q = 1; q < NW; ++q)
                 if (better(s_wv[q], s_wi[q], bv, bi)) { bv = s_wv[q]; bi = s_wi[q]; }
-            slot_val[par * G + w] = bv;
-            wk.slot_idx[par * G + w] = bi;
+            s_bv = bv;
             s_p = bi;
         }
         __syncthreads();
-        // -- publish the candidate row and (owner of row k) the diagonal row
+        PANEL_TRACE(k, 1);
+        // -- publish the candidate row and (slab 0) row k as tagged words --
         if (own && r == s_p) {
-            T* dst = slot_rows + ((i64)par * G + w) * NB;
 #pragma unroll
-            for (int c = 0; c < NB; ++c) dst[c] = row[c];
+            for (int c = 0; c < NB; ++c) s_out[0][c] = row[c];
         }
         if (own && r == k) {
-            T* dst = diag_rows + (i64)par * NB;
 #pragma unroll
-            for (int c = 0; c < NB; ++c) dst[c] = row[c];
-        }
-        if (!grid_barrier(wk.ctr, base + (unsigned)(G * (k + 1)), wk.err)) {
-            alive = false;
-            break;
-        }
-        // -- global winner (identical in every workgroup) -----------------
-        if (wave == 0) {
-            T bv = T(-1); int bi = INT_MAX, bw = 0;
-            for (int q = lane; q < G; q += 64) {
-                T qv = slot_val[par * G + q];
-                int qi = wk.slot_idx[par * G + q];
-                if (better(qv, qi, bv, bi)) { bv = qv; bi = qi; bw = q; }
-            }
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                T v2 = __shfl_xor(bv, off, 64);
-                int i2 = __shfl_xor(bi, off, 64);
-                int w2 = __shfl_xor(bw, off, 64);
-                if (better(v2, i2, bv, bi)) { bv = v2; bi = i2; bw = w2; }
-            }
-            if (lane == 0) {
-                const bool zero = !(bv >= tiny<T>());   // TMV_Underflow
-                s_zero = zero ? 1 : 0;
-                s_p = zero ? k : bi;
-                s_win = bw;
-            }
+            for (int c = 0; c < NB; ++c) s_out[1][c] = row[c];
         }
         __syncthreads();
+        for (int e = t; e < SL::SIZE; e += NTH) {
+            unsigned pay;
+            if (e < SL::DIAG)      pay = half_bits(s_out[0][e / WPV], e % WPV);
+            else if (e < SL::VAL) {
+                if (w != 0) continue;                       // row k: slab 0 only
+                pay = half_bits(s_out[1][(e - SL::DIAG) / WPV], (e - SL::DIAG) % WPV);
+            }
+            else if (e < SL::IDX)  pay = half_bits(s_bv, e - SL::VAL);
+            else                   pay = (unsigned)s_p;
+            cstore(myslot + e, tag | pay);
+        }
+        PANEL_TRACE(k, 2);
+        // -- receive every slab's words once they carry this column's tag --
+        {
+            const int nrow = G * SL::DIAG;               // candidate-row words
+            const int total = nrow + SL::DIAG + G * (WPV + 1);
+            unsigned spins = 0;
+            bool ok = true;
+            for (int e = t; e < total && ok; e += NTH) {
+                const unsigned long long* src;
+                if (e < nrow) {
+                    src = wk.slots + ((i64)par * G + e / SL::DIAG) * SL::SIZE + SL::ROW + e % SL::DIAG;
+                } else if (e < nrow + SL::DIAG) {
+                    src = wk.slots + (i64)par * G * SL::SIZE + SL::DIAG + (e - nrow);
+                } else {
+                    const int f = e - nrow - SL::DIAG, q = f / (WPV + 1), h = f % (WPV + 1);
+                    src = wk.slots + ((i64)par * G + q) * SL::SIZE + SL::VAL + h;
+                }
+                unsigned long long word;
+                while (((word = cload(src)) & ~0xffffffffull) != tag) {
+                    if ((++spins & 255) == 0 &&
+                        (spins > SPIN_LIMIT ||
+                         __hip_atomic_load(wk.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                        __hip_atomic_store(wk.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        s_fail = 1;
+                        ok = false;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                if (!ok) break;
+                const unsigned pay = (unsigned)word;
+                if (e < nrow) s_cand[e / SL::DIAG][e % SL::DIAG] = pay;
+                else if (e < nrow + SL::DIAG) s_diag[e - nrow] = pay;
+                else {
+                    const int f = e - nrow - SL::DIAG, q = f / (WPV + 1), h = f % (WPV + 1);
+                    if (h < WPV) s_cval[q][h] = pay; else s_cidx[q] = (int)pay;
+                }
+            }
+        }
+        PANEL_TRACE(k, 3);
+        __syncthreads();
+        PANEL_TRACE(k, 4);
+        if (s_fail) { alive = false; break; }
+        // -- global winner (identical in every workgroup) -----------------
+        if (t == 0) {
+            T bv = T(-1); int bi = INT_MAX, bw = 0;
+            for (int q = 0; q < G; ++q) {
+                const T qv = from_halves<T>(s_cval[q]);
+                if (better(qv, s_cidx[q], bv, bi)) { bv = qv; bi = s_cidx[q]; bw = q; }
+            }
+            const bool zero = !(bv >= tiny<T>());   // TMV_Underflow
+            s_zero = zero ? 1 : 0;
+            s_p = zero ? k : bi;
+            s_win = bw;
+        }
+        __syncthreads();
+        if (t < NB) s_prow[t] = from_halves<T>(&s_cand[s_win][t * WPV]);
+        else if (t < 2 * NB) s_orow[t - NB] = from_halves<T>(&s_diag[(t - NB) * WPV]);
+        __syncthreads();
+        PANEL_TRACE(k, 5);
         const int p = s_p;
         if (s_zero) {
             if (own && r >= k) row[0] = T(0);
@@ -194,11 +305,6 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
                           (unsigned long long)(j0 + k + 1));
             }
         } else {
-            if (t < NB) {
-                s_prow[t] = slot_rows[((i64)par * G + s_win) * NB + t];
-                if (p != k) s_orow[t] = diag_rows[(i64)par * NB + t];
-            }
-            __syncthreads();
             if (own && p != k && r == p) {
 #pragma unroll
                 for (int c = 0; c < NB; ++c) row[c] = s_orow[c];
@@ -218,10 +324,12 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
             }
             if (w == 0 && t == 0) piv[j0 + k] = j0 + p;
         }
+        PANEL_TRACE(k, 6);
         rotate();
         ++nrot;
         cv = own ? fabs(row[0]) : T(-1);
         __syncthreads();
+        PANEL_TRACE(k, 7);
     }
     // Restore column order (jb < NB leaves the rows part-way round).
     for (int q = nrot % NB; q != 0 && q < NB; ++q) rotate();
@@ -231,29 +339,90 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
         if (own && c < jb) gp[(i64)c * lda] = row[c];
 }
 
-// Interchanges piv[k1..k2) applied to columns [0, c1) and [c2, ncols) of A
-// (all columns when c1 == c2).  forward: k ascending (permuteRows);
-// otherwise descending (reversePermuteRows).
-template <typename T>
-__global__ void laswp_kernel(T* __restrict__ A, i64 lda, i64 c1, i64 c2,
-                             i64 ncols, i64 k1, i64 k2,
-                             const i64* __restrict__ piv, int forward, i64 nrows)
+// Net effect of interchanges piv[j0 .. j0+jb) (absolute rows, each >= its
+// step): new row j0+i = old row src[i] (i < jb) and new row odst[q] = old
+// row osrc[q] for the nout rows below the block that took part.  Computed
+// by one wave: lane i holds src[i] and the i-th outside row, lookups of an
+// outside row are a ballot.  Out-of-range pivots are ignored (no access).
+template <int NB>
+struct PanelPerm {
+    int nout;
+    i64 src[NB];
+    i64 odst[NB], osrc[NB];
+};
+
+template <int NB>
+__device__ void panel_perm_wave(PanelPerm<NB>& P, const i64* lpiv, i64 j0, int jb, i64 nrows)
 {
-    const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    const i64 nleft = c1, nright = ncols - c2;
-    if (idx >= nleft + nright) return;
-    const i64 col = idx < nleft ? idx : c2 + (idx - nleft);
-    T* a = A + col * lda;
-    if (forward) {
-        for (i64 k = k1; k < k2; ++k) {
-            const i64 p = piv[k];
-            if (p != k && p >= 0 && p < nrows) { T x = a[k]; a[k] = a[p]; a[p] = x; }
+    static_assert(NB == 64, "one lane per panel column");
+    const int lane = threadIdx.x & 63;
+    long long src = j0 + lane, odst = -1, osrc = -1;
+    int nout = 0;
+    for (int k = 0; k < jb; ++k) {
+        const i64 p = lpiv[k];
+        if (p <= j0 + k || p >= nrows) continue;            // uniform
+        if (p < j0 + jb) {
+            const int kp = (int)(p - j0);
+            const long long a = __shfl(src, k, 64), b = __shfl(src, kp, 64);
+            if (lane == k) src = b;
+            if (lane == kp) src = a;
+        } else {
+            const unsigned long long hit = __ballot(lane < nout && odst == p);
+            int q;
+            if (hit) q = __ffsll((long long)hit) - 1;
+            else {
+                q = nout++;
+                if (lane == q) { odst = p; osrc = p; }
+            }
+            const long long a = __shfl(src, k, 64), b = __shfl(osrc, q, 64);
+            if (lane == k) src = b;
+            if (lane == q) osrc = a;
         }
-    } else {
-        for (i64 k = k2 - 1; k >= k1; --k) {
-            const i64 p = piv[k];
-            if (p != k && p >= 0 && p < nrows) { T x = a[k]; a[k] = a[p]; a[p] = x; }
-        }
+    }
+    P.src[lane] = src;
+    P.odst[lane] = odst;
+    P.osrc[lane] = osrc;
+    if (lane == 0) P.nout = nout;
+}
+
+// Apply the panel's net permutation to columns [0, c1) and [c2, ncols) of A
+// (one wave per column: all loads, then all stores), and to the composed
+// permutation vector `perm` (when given) by the first wave of workgroup 0.
+template <typename T, int NB>
+__global__ void __launch_bounds__(256)
+laswp_panel_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 c1, i64 c2, i64 ncols,
+                   i64 j0, int jb, const i64* __restrict__ piv, i64* __restrict__ perm,
+                   int cols_per_wave)
+{
+    __shared__ PanelPerm<NB> P;
+    __shared__ i64 lpiv[NB];
+    if (threadIdx.x < jb) lpiv[threadIdx.x] = piv[j0 + threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x < 64) panel_perm_wave<NB>(P, lpiv, j0, jb, nrows);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int nout = P.nout;
+    if (perm != nullptr && blockIdx.x == 0 && threadIdx.x < 64) {
+        i64 v1 = 0, v2 = 0;
+        if (lane < jb) v1 = perm[P.src[lane]];
+        if (lane < nout) v2 = perm[P.osrc[lane]];
+        __builtin_amdgcn_s_waitcnt(0);     // every load returned before any store
+        if (lane < jb) perm[j0 + lane] = v1;
+        if (lane < nout) perm[P.odst[lane]] = v2;
+    }
+    const i64 nleft = c1, nright = ncols - c2, total = nleft + nright;
+    const i64 wave = (i64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    for (int q = 0; q < cols_per_wave; ++q) {
+        const i64 idx = wave * cols_per_wave + q;
+        if (idx >= total) break;
+        const i64 col = idx < nleft ? idx : c2 + (idx - nleft);
+        T* a = A + col * lda;
+        T v1 = T(0), v2 = T(0);
+        if (lane < jb) v1 = a[P.src[lane]];
+        if (lane < nout) v2 = a[P.osrc[lane]];
+        __builtin_amdgcn_s_waitcnt(0);     // every load returned before any store
+        if (lane < jb) a[j0 + lane] = v1;
+        if (lane < nout) a[P.odst[lane]] = v2;
     }
 }
 
@@ -300,61 +469,72 @@ trsm_u12_kernel(T* __restrict__ A, i64 lda, i64 j0, int jb, i64 ncols, int cpw)
     }
 }
 
-__global__ void reset_counter_kernel(unsigned* ctr, int* err, i64* first_zero, i64 n)
+__global__ void getrf_init_kernel(unsigned long long* tags, i64 ntags, int* err,
+                                  i64* first_zero, i64 n, i64* perm)
 {
-    *ctr = 0u;
-    *err = 0;
-    *first_zero = n + 1;
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ntags) tags[i] = 0ull;
+    if (i == 0) { *err = 0; *first_zero = n + 1; }
+    if (perm != nullptr && i < n) perm[i] = i;
+}
+
+__global__ void zero_kernel(unsigned char* p, size_t bytes)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < bytes) p[i] = 0;
+}
+
+int zero_bytes(void* p, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return TMVX_OK;
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((bytes + 255) / 256)), dim3(256), 0, s,
+                       static_cast<unsigned char*>(p), bytes);
+    return hip_status(hipGetLastError());
 }
 
 template <typename T>
-int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* info, hipStream_t s)
+int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s)
 {
     if (n < 0 || lda < (n > 0 ? n : 1) || !info) return TMVX_ERR_ARG;
     *info = 0;
     if (n == 0) return TMVX_OK;
     if (!A || !piv) return TMVX_ERR_ARG;
-    if (n > (i64)INT_MAX / 2) return TMVX_ERR_UNSUPPORTED;
     constexpr int NB = PANEL_NB, NTH = PANEL_THREADS;
     const int Gmax = (int)((n + NTH - 1) / NTH);
-    if (Gmax > 256) return TMVX_ERR_UNSUPPORTED;   // > 131072 rows
+    if (Gmax > PANEL_GMAX) return TMVX_ERR_UNSUPPORTED;   // n > 32768
 
-    // Workspace: slots + counter + flags, one allocation.
-    const size_t szrows = sizeof(T) * 2 * (size_t)Gmax * NB;
-    const size_t szval = sizeof(T) * 2 * (size_t)Gmax;
-    const size_t szidx = sizeof(int) * 2 * (size_t)Gmax;
-    const size_t szdiag = sizeof(T) * 2 * NB;
-    const size_t total = szrows + szval + szidx + szdiag + 256;
+    const i64 nslot = 2 * (i64)Gmax * SlotLayout<T, NB>::SIZE;
+    const size_t szslots = sizeof(unsigned long long) * (size_t)nslot;
+    const size_t total = szslots + 256;
     char* ws = nullptr;
     if (hipMallocAsync((void**)&ws, total, s) != hipSuccess) return TMVX_ERR_ALLOC;
     PanelWork wk;
-    wk.slot_rows = ws;
-    wk.slot_val = ws + szrows;
-    wk.slot_idx = reinterpret_cast<int*>(ws + szrows + szval);
-    wk.diag_rows = ws + szrows + szval + szidx;
-    char* tail = ws + szrows + szval + szidx + szdiag;
-    tail = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(tail) + 63) & ~uintptr_t(63));
-    wk.ctr = reinterpret_cast<unsigned*>(tail);
-    wk.err = reinterpret_cast<int*>(tail + 64);
-    wk.first_zero = reinterpret_cast<i64*>(tail + 128);
+    wk.slots = reinterpret_cast<unsigned long long*>(ws);
+    char* tail = ws + szslots;
+    wk.err = reinterpret_cast<int*>(tail);
+    wk.first_zero = reinterpret_cast<i64*>(tail + 64);
 
     int rc = TMVX_OK;
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, wk.ctr, wk.err,
-                       wk.first_zero, n);
-    unsigned base = 0;
+    {
+        const i64 cnt = n > nslot ? n : nslot;
+        hipLaunchKernelGGL(getrf_init_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256),
+                           0, s, wk.slots, nslot, wk.err, wk.first_zero, n, perm);
+        rc = hip_status(hipGetLastError());
+    }
     for (i64 j0 = 0; j0 < n && rc == TMVX_OK; j0 += NB) {
         const int jb = (int)((n - j0) < NB ? (n - j0) : NB);
         const int G = (int)((n - j0 + NTH - 1) / NTH);
         hipLaunchKernelGGL((panel_kernel<T, NB, NTH>), dim3(G), dim3(NTH), 0, s,
-                           A, lda, n, j0, jb, piv, wk, base);
-        if (hipGetLastError() != hipSuccess) { rc = TMVX_ERR_HIP; break; }
-        base += (unsigned)(G * jb);
+                           A, lda, n, j0, jb, piv, wk);
+        if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
         const i64 nother = j0 + (n - j0 - jb);
-        if (nother > 0) {
-            hipLaunchKernelGGL(laswp_kernel<T>, dim3((unsigned)((nother + 255) / 256)),
-                               dim3(256), 0, s, A, lda, j0, j0 + jb, n, j0, j0 + jb,
-                               piv, 1, n);
-            if (hipGetLastError() != hipSuccess) { rc = TMVX_ERR_HIP; break; }
+        if (nother > 0 || perm) {
+            const int cpw = 4;
+            const i64 waves = (nother + cpw - 1) / cpw;
+            const unsigned grid = (unsigned)(waves > 0 ? (waves + 3) / 4 : 1);
+            hipLaunchKernelGGL((laswp_panel_kernel<T, NB>), dim3(grid), dim3(256), 0, s, A, lda,
+                               n, j0, j0 + jb, n, j0, jb, piv, perm, cpw);
+            if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
         }
         const i64 nr = n - j0 - jb;
         if (nr > 0) {
@@ -362,7 +542,7 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* info, hipStream_t s)
             const i64 waves = (nr + cpw - 1) / cpw;
             hipLaunchKernelGGL((trsm_u12_kernel<T, NB>), dim3((unsigned)((waves + 3) / 4)),
                                dim3(256), 0, s, A, lda, j0, jb, nr, cpw);
-            if (hipGetLastError() != hipSuccess) { rc = TMVX_ERR_HIP; break; }
+            if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
             rc = gemm_launch<T>(0, 0, nr, nr, jb, T(-1), A + (j0 + jb) + j0 * lda, lda,
                                 A + j0 + (j0 + jb) * lda, lda, T(1),
                                 A + (j0 + jb) + (j0 + jb) * lda, lda, s);
@@ -381,249 +561,57 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* info, hipStream_t s)
     return TMVX_OK;
 }
 
-// ---------------------------------------------------------------------------
-// Triangular solves: blocked substitution, one launch per diagonal block.
-// ---------------------------------------------------------------------------
-//
-// Solves op(A) X = B for op(A) lower (forward) or upper (backward)
-// triangular.  W holds the running right-hand side (a copy of B), B gets
-// X block by block.  Step kernel for diagonal block [b0, b1):
-//   - every workgroup loads the diagonal block of op(A) and W[b0:b1, :] into
-//     LDS and solves it by substitution (one wave per RHS column; a zero
-//     diagonal met by a non-zero entry sets *singular, as TMV throws);
-//   - workgroup 0 writes X_b into B[b0:b1, :];
-//   - each workgroup updates its slab of the remaining rows:
-//     W[i, :] -= op(A)[i, b0:b1] * X_b.
-constexpr int TRSM_NB = 32;
-constexpr int TRSM_ROWS = 256;   // trailing rows per workgroup (one per thread)
+// Interchange list -> composed permutation, one thread over an LDS array
+// (n <= PERM_LDS_MAX), else over global memory.
+constexpr int PERM_LDS_MAX = 32768;
 
-template <typename T, int NB>
-__global__ void __launch_bounds__(256)
-trsm_step_kernel(int lower, int trans, int unit, i64 n, i64 nrhs,
-                 const T* __restrict__ A, i64 lda, T* __restrict__ W, i64 ldw,
-                 T* __restrict__ B, i64 ldb, i64 b0, int bs, int* singular)
+__global__ void __launch_bounds__(1024)
+piv_to_perm_kernel(i64 n, const i64* __restrict__ piv, i64* __restrict__ perm)
 {
-    __shared__ T D[NB][NB + 1];
-    __shared__ T X[NB][NB + 1];       // X[i][jj] for a chunk of NB RHS columns
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    auto opA = [&](i64 i, i64 j) -> T {
-        return trans ? A[j + i * lda] : A[i + j * lda];
-    };
-    for (int e = t; e < NB * NB; e += blockDim.x) {
-        const int i = e % NB, j = e / NB;
-        D[i][j] = (i < bs && j < bs) ? opA(b0 + i, b0 + j) : T(0);
+    __shared__ int sp[PERM_LDS_MAX];
+    const bool lds = n <= PERM_LDS_MAX;
+    for (i64 i = threadIdx.x; i < n; i += blockDim.x) {
+        if (lds) sp[i] = (int)i; else perm[i] = i;
     }
-    // Trailing rows handled by this workgroup.
-    const i64 trail_lo = lower ? b0 + bs : 0;
-    const i64 trail_hi = lower ? n : b0;
-    const i64 row = trail_lo + (i64)blockIdx.x * TRSM_ROWS + t;
-    const bool has_row = row < trail_hi;
-    T arow[NB];
-#pragma unroll
-    for (int k = 0; k < NB; ++k)
-        arow[k] = (has_row && k < bs) ? opA(row, b0 + k) : T(0);
-
-    for (i64 j0 = 0; j0 < nrhs; j0 += NB) {
-        const int jn = (int)((nrhs - j0) < NB ? (nrhs - j0) : NB);
-        __syncthreads();
-        for (int e = t; e < NB * NB; e += blockDim.x) {
-            const int i = e % NB, jj = e / NB;
-            X[i][jj] = (i < bs && jj < jn) ? W[(b0 + i) + (j0 + jj) * ldw] : T(0);
-        }
-        __syncthreads();
-        // Substitution: wave handles columns jj = wave, wave+4, ...
-        for (int jj = wave; jj < jn; jj += 4) {
-            T x = lane < bs ? X[lane][jj] : T(0);
-            if (lower) {
-                for (int k = 0; k < bs; ++k) {
-                    T xk = __shfl(x, k, 64);
-                    if (!unit && xk != T(0)) {
-                        const T d = D[k][k];
-                        if (d == T(0)) { if (lane == 0) atomicOr(singular, 1); }
-                        else xk = xk / d;
-                        if (lane == k) x = xk;
-                    }
-                    if (lane > k && lane < bs) x -= D[lane][k] * xk;
-                }
-            } else {
-                for (int k = bs - 1; k >= 0; --k) {
-                    T xk = __shfl(x, k, 64);
-                    if (!unit && xk != T(0)) {
-                        const T d = D[k][k];
-                        if (d == T(0)) { if (lane == 0) atomicOr(singular, 1); }
-                        else xk = xk / d;
-                        if (lane == k) x = xk;
-                    }
-                    if (lane < k) x -= D[lane][k] * xk;
-                }
-            }
-            if (lane < bs) X[lane][jj] = x;
-        }
-        __syncthreads();
-        if (blockIdx.x == 0) {
-            for (int e = t; e < NB * NB; e += blockDim.x) {
-                const int i = e % NB, jj = e / NB;
-                if (i < bs && jj < jn) B[(b0 + i) + (j0 + jj) * ldb] = X[i][jj];
-            }
-        }
-        if (has_row) {
-            for (int jj = 0; jj < jn; ++jj) {
-                T s = W[row + (j0 + jj) * ldw];
-#pragma unroll
-                for (int k = 0; k < NB; ++k) s -= arow[k] * X[k][jj];
-                W[row + (j0 + jj) * ldw] = s;
-            }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (i64 k = 0; k < n; ++k) {
+            const i64 p = piv[k];
+            if (p == k || p < 0 || p >= n) continue;
+            if (lds) { const int x = sp[k]; sp[k] = sp[p]; sp[p] = x; }
+            else { const i64 x = perm[k]; perm[k] = perm[p]; perm[p] = x; }
         }
     }
+    __syncthreads();
+    if (lds)
+        for (i64 i = threadIdx.x; i < n; i += blockDim.x) perm[i] = sp[i];
 }
 
-template <typename T>
-__global__ void copy_kernel(i64 m, i64 n, const T* __restrict__ S, i64 lds,
-                            T* __restrict__ D, i64 ldd)
+int piv_to_perm(i64 n, const i64* piv, i64* perm, hipStream_t s)
 {
-    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    const i64 j = blockIdx.y;
-    if (i < m && j < n) D[i + j * ldd] = S[i + j * lds];
+    if (n <= 0) return TMVX_OK;
+    hipLaunchKernelGGL(piv_to_perm_kernel, dim3(1), dim3(1024), 0, s, n, piv, perm);
+    return hip_status(hipGetLastError());
 }
 
-template <typename T>
-int trsm_impl(int upper, int trans, int unit, i64 n, i64 nrhs, const T* A,
-              i64 lda, T* B, i64 ldb, int* d_singular, hipStream_t s)
-{
-    if (n < 0 || nrhs < 0 || lda < (n > 0 ? n : 1) || ldb < (n > 0 ? n : 1))
-        return TMVX_ERR_ARG;
-    if (n == 0 || nrhs == 0) return TMVX_OK;
-    if (!A || !B) return TMVX_ERR_ARG;
-    if (nrhs > 65535) return TMVX_ERR_UNSUPPORTED;
-    const int lower = (upper == 0) != (trans != 0);   // op(A) lower?
-    T* W = nullptr;
-    if (hipMallocAsync((void**)&W, sizeof(T) * (size_t)n * (size_t)nrhs, s) != hipSuccess)
-        return TMVX_ERR_ALLOC;
-    hipLaunchKernelGGL(copy_kernel<T>, dim3((unsigned)((n + 255) / 256), (unsigned)nrhs),
-                       dim3(256), 0, s, n, nrhs, B, ldb, W, n);
-    int rc = hip_status(hipGetLastError());
-    constexpr int NB = TRSM_NB;
-    const i64 nblk = (n + NB - 1) / NB;
-    for (i64 q = 0; q < nblk && rc == TMVX_OK; ++q) {
-        const i64 blk = lower ? q : nblk - 1 - q;
-        const i64 b0 = blk * NB;
-        const int bs = (int)((n - b0) < NB ? (n - b0) : NB);
-        const i64 ntrail = lower ? n - b0 - bs : b0;
-        const unsigned grid = (unsigned)(ntrail > 0 ? (ntrail + TRSM_ROWS - 1) / TRSM_ROWS : 1);
-        hipLaunchKernelGGL((trsm_step_kernel<T, NB>), dim3(grid), dim3(256), 0, s, lower,
-                           trans, unit, n, nrhs, A, lda, W, n, B, ldb, b0, bs, d_singular);
-        rc = hip_status(hipGetLastError());
-    }
-    hipFreeAsync(W, s);
-    return rc;
-}
-
-template <typename T>
-int getrs_impl(int trans, i64 n, i64 nrhs, const T* LU, i64 lda, const i64* piv,
-               T* B, i64 ldb, int check, hipStream_t s)
-{
-    if (n < 0 || nrhs < 0 || lda < (n > 0 ? n : 1) || ldb < (n > 0 ? n : 1))
-        return TMVX_ERR_ARG;
-    if (n == 0 || nrhs == 0) return TMVX_OK;
-    if (!LU || !piv || !B) return TMVX_ERR_ARG;
-    int* d_sing = nullptr;
-    if (hipMallocAsync((void**)&d_sing, sizeof(int), s) != hipSuccess) return TMVX_ERR_ALLOC;
-    if (hipMemsetAsync(d_sing, 0, sizeof(int), s) != hipSuccess) return TMVX_ERR_HIP;
-    int rc = TMVX_OK;
-    const unsigned gcols = (unsigned)((nrhs + 255) / 256);
-    if (!trans) {
-        hipLaunchKernelGGL(laswp_kernel<T>, dim3(gcols), dim3(256), 0, s, B, ldb, (i64)0,
-                           (i64)0, nrhs, (i64)0, n, piv, 1, n);
-        rc = hip_status(hipGetLastError());
-        if (rc == TMVX_OK) rc = trsm_impl<T>(0, 0, 1, n, nrhs, LU, lda, B, ldb, d_sing, s);
-        if (rc == TMVX_OK) rc = trsm_impl<T>(1, 0, 0, n, nrhs, LU, lda, B, ldb, d_sing, s);
-    } else {
-        rc = trsm_impl<T>(1, 1, 0, n, nrhs, LU, lda, B, ldb, d_sing, s);
-        if (rc == TMVX_OK) rc = trsm_impl<T>(0, 1, 1, n, nrhs, LU, lda, B, ldb, d_sing, s);
-        if (rc == TMVX_OK) {
-            hipLaunchKernelGGL(laswp_kernel<T>, dim3(gcols), dim3(256), 0, s, B, ldb, (i64)0,
-                               (i64)0, nrhs, (i64)0, n, piv, 0, n);
-            rc = hip_status(hipGetLastError());
-        }
-    }
-    int hs = 0;
-    if (check && rc == TMVX_OK) {
-        if (hipMemcpyAsync(&hs, d_sing, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess)
-            rc = TMVX_ERR_HIP;
-    }
-    hipFreeAsync(d_sing, s);
-    if (check) {
-        if (hipStreamSynchronize(s) != hipSuccess) return TMVX_ERR_HIP;
-        if (rc == TMVX_OK && hs) return TMVX_ERR_SINGULAR;
-    }
-    return rc;
-}
-
-template int getrf_impl<double>(i64, double*, i64, i64*, i64*, hipStream_t);
-template int getrf_impl<float>(i64, float*, i64, i64*, i64*, hipStream_t);
-template int getrs_impl<double>(int, i64, i64, const double*, i64, const i64*, double*, i64, int, hipStream_t);
-template int getrs_impl<float>(int, i64, i64, const float*, i64, const i64*, float*, i64, int, hipStream_t);
+template int getrf_impl<double>(i64, double*, i64, i64*, i64*, i64*, hipStream_t);
+template int getrf_impl<float>(i64, float*, i64, i64*, i64*, i64*, hipStream_t);
 
 } // namespace tmvx
 
-using tmvx::i64;
+#ifdef TMVX_PANEL_TRACE
+extern "C" int tmvx_panel_trace_set(unsigned long long* buf)
+{
+    return hipMemcpyToSymbol(HIP_SYMBOL(tmvx::g_panel_trace), &buf, sizeof(buf)) == hipSuccess
+               ? 0 : -3;
+}
+#endif
 
 extern "C" {
 
 int tmvx_getrf_f64(int64_t n, double* A, int64_t lda, int64_t* piv, int64_t* info, void* stream)
-{ return tmvx::getrf_impl<double>(n, A, lda, piv, info, (hipStream_t)stream); }
+{ return tmvx::getrf_impl<double>(n, A, lda, piv, nullptr, info, (hipStream_t)stream); }
 int tmvx_getrf_f32(int64_t n, float* A, int64_t lda, int64_t* piv, int64_t* info, void* stream)
-{ return tmvx::getrf_impl<float>(n, A, lda, piv, info, (hipStream_t)stream); }
-
-int tmvx_getrs_f64(int trans, int64_t n, int64_t nrhs, const double* LU, int64_t lda,
-                   const int64_t* piv, double* B, int64_t ldb, int check, void* stream)
-{ return tmvx::getrs_impl<double>(trans, n, nrhs, LU, lda, piv, B, ldb, check, (hipStream_t)stream); }
-int tmvx_getrs_f32(int trans, int64_t n, int64_t nrhs, const float* LU, int64_t lda,
-                   const int64_t* piv, float* B, int64_t ldb, int check, void* stream)
-{ return tmvx::getrs_impl<float>(trans, n, nrhs, LU, lda, piv, B, ldb, check, (hipStream_t)stream); }
-
-int tmvx_trsm_f64(int upper, int trans, int unit, int64_t n, int64_t nrhs, const double* A,
-                  int64_t lda, double* B, int64_t ldb, void* stream)
-{
-    hipStream_t s = (hipStream_t)stream;
-    int* d = nullptr;
-    if (hipMallocAsync((void**)&d, sizeof(int), s) != hipSuccess) return TMVX_ERR_ALLOC;
-    if (hipMemsetAsync(d, 0, sizeof(int), s) != hipSuccess) return TMVX_ERR_HIP;
-    int rc = tmvx::trsm_impl<double>(upper, trans, unit, n, nrhs, A, lda, B, ldb, d, s);
-    hipFreeAsync(d, s);
-    return rc;
-}
-int tmvx_trsm_f32(int upper, int trans, int unit, int64_t n, int64_t nrhs, const float* A,
-                  int64_t lda, float* B, int64_t ldb, void* stream)
-{
-    hipStream_t s = (hipStream_t)stream;
-    int* d = nullptr;
-    if (hipMallocAsync((void**)&d, sizeof(int), s) != hipSuccess) return TMVX_ERR_ALLOC;
-    if (hipMemsetAsync(d, 0, sizeof(int), s) != hipSuccess) return TMVX_ERR_HIP;
-    int rc = tmvx::trsm_impl<float>(upper, trans, unit, n, nrhs, A, lda, B, ldb, d, s);
-    hipFreeAsync(d, s);
-    return rc;
-}
-
-int tmvx_laswp_f64(int forward, int64_t ncols, double* A, int64_t lda, int64_t k1, int64_t k2,
-                   const int64_t* piv, void* stream)
-{
-    if (ncols < 0 || k1 < 0 || k2 < k1 || lda < 1) return TMVX_ERR_ARG;
-    if (ncols == 0 || k1 == k2) return TMVX_OK;
-    hipLaunchKernelGGL(tmvx::laswp_kernel<double>, dim3((unsigned)((ncols + 255) / 256)),
-                       dim3(256), 0, (hipStream_t)stream, A, lda, (i64)0, (i64)0, ncols, k1,
-                       k2, piv, forward, lda);
-    return tmvx::hip_status(hipGetLastError());
-}
-int tmvx_laswp_f32(int forward, int64_t ncols, float* A, int64_t lda, int64_t k1, int64_t k2,
-                   const int64_t* piv, void* stream)
-{
-    if (ncols < 0 || k1 < 0 || k2 < k1 || lda < 1) return TMVX_ERR_ARG;
-    if (ncols == 0 || k1 == k2) return TMVX_OK;
-    hipLaunchKernelGGL(tmvx::laswp_kernel<float>, dim3((unsigned)((ncols + 255) / 256)),
-                       dim3(256), 0, (hipStream_t)stream, A, lda, (i64)0, (i64)0, ncols, k1,
-                       k2, piv, forward, lda);
-    return tmvx::hip_status(hipGetLastError());
-}
+{ return tmvx::getrf_impl<float>(n, A, lda, piv, nullptr, info, (hipStream_t)stream); }
 
 } // extern "C"
