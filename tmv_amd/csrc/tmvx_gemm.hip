@@ -40,14 +40,19 @@ template <> struct GemmTable<float> {
 
 // Default configuration by shape (chosen from measurements on MI355X).
 template <typename T> int default_cfg(int ta, int tb, i64 m, i64 n, i64 k);
+// Short K (the LU updates: K = 64 inside an outer panel, K = 256 for the
+// trailing matrix) favours small tiles: the C read-modify-write dominates
+// and occupancy hides it (measured on MI355X at M = N = 4000..16000).
 template <> int default_cfg<double>(int ta, int tb, i64 m, i64 n, i64 k)
 {
     if (m * n < 1024 * 1024) return (m * n <= 256 * 256) ? 4 : 0;
+    if (k <= 128) return 0;
     return (!ta && !tb) ? 1 : 0;
 }
 template <> int default_cfg<float>(int ta, int tb, i64 m, i64 n, i64 k)
 {
-    if (m * n < 1024 * 1024) return 3;
+    if (m * n < 1024 * 1024 || k <= 128) return 3;
+    if (k <= 512) return 4;
     if (!ta && !tb) return 0;
     if (ta && !tb) return 2;
     return 1;

@@ -385,12 +385,12 @@ __device__ void panel_perm_wave(PanelPerm<NB>& P, const i64* lpiv, i64 j0, int j
     if (lane == 0) P.nout = nout;
 }
 
-// Apply the panel's net permutation to columns [0, c1) and [c2, ncols) of A
+// Apply the panel's net permutation to columns [a0, a1) and [b0, b1) of A
 // (one wave per column: all loads, then all stores), and to the composed
 // permutation vector `perm` (when given) by the first wave of workgroup 0.
 template <typename T, int NB>
 __global__ void __launch_bounds__(256)
-laswp_panel_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 c1, i64 c2, i64 ncols,
+laswp_panel_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 a0, i64 a1, i64 b0, i64 b1,
                    i64 j0, int jb, const i64* __restrict__ piv, i64* __restrict__ perm,
                    int cols_per_wave)
 {
@@ -410,12 +410,12 @@ laswp_panel_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 c1, i64 c2, i64 nc
         if (lane < jb) perm[j0 + lane] = v1;
         if (lane < nout) perm[P.odst[lane]] = v2;
     }
-    const i64 nleft = c1, nright = ncols - c2, total = nleft + nright;
+    const i64 nleft = a1 - a0, nright = b1 - b0, total = nleft + nright;
     const i64 wave = (i64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     for (int q = 0; q < cols_per_wave; ++q) {
         const i64 idx = wave * cols_per_wave + q;
         if (idx >= total) break;
-        const i64 col = idx < nleft ? idx : c2 + (idx - nleft);
+        const i64 col = idx < nleft ? a0 + idx : b0 + (idx - nleft);
         T* a = A + col * lda;
         T v1 = T(0), v2 = T(0);
         if (lane < jb) v1 = a[P.src[lane]];
@@ -427,7 +427,7 @@ laswp_panel_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 c1, i64 c2, i64 nc
 }
 
 // U12 = L11^-1 A12, L11 = unit-lower A[j0:j0+jb, j0:j0+jb],
-// A12 = A[j0:j0+jb, j0+jb : j0+jb+ncols].  Lane i of a wave owns row i:
+// A12 = A[j0:j0+jb, c0 : c0+ncols].  Lane i of a wave owns row i:
 // it keeps row i of L11 in registers and, column by column, forward-
 // substitutes with x_k broadcast by readlane.  Each wave solves CW columns
 // at once (independent chains interleave) and walks `cpw` columns.
@@ -435,7 +435,7 @@ constexpr int U12_CW = 4;
 
 template <typename T, int NB>
 __global__ void __launch_bounds__(256)
-trsm_u12_kernel(T* __restrict__ A, i64 lda, i64 j0, int jb, i64 ncols, int cpw)
+trsm_u12_kernel(T* __restrict__ A, i64 lda, i64 j0, int jb, i64 cbeg, i64 ncols, int cpw)
 {
     static_assert(NB == 64, "one lane per panel row");
     const int lane = threadIdx.x & 63;
@@ -449,7 +449,7 @@ trsm_u12_kernel(T* __restrict__ A, i64 lda, i64 j0, int jb, i64 ncols, int cpw)
         constexpr int k = decltype(kc)::value;
         Lrow[k] = (lane < jb && k < lane) ? arow[(i64)k * lda] : T(0);
     });
-    T* bcol = A + (j0 + lane) + (j0 + jb) * lda;
+    T* bcol = A + (j0 + lane) + cbeg * lda;
     for (i64 c = c0; c < c1; c += U12_CW) {
         T x[U12_CW];
 #pragma unroll
@@ -521,32 +521,71 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
                            0, s, wk.slots, nslot, wk.err, wk.first_zero, n, perm);
         rc = hip_status(hipGetLastError());
     }
-    for (i64 j0 = 0; j0 < n && rc == TMVX_OK; j0 += NB) {
-        const int jb = (int)((n - j0) < NB ? (n - j0) : NB);
-        const int G = (int)((n - j0 + NTH - 1) / NTH);
-        hipLaunchKernelGGL((panel_kernel<T, NB, NTH>), dim3(G), dim3(NTH), 0, s,
-                           A, lda, n, j0, jb, piv, wk);
-        if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
-        const i64 nother = j0 + (n - j0 - jb);
-        if (nother > 0 || perm) {
-            const int cpw = 4;
-            const i64 waves = (nother + cpw - 1) / cpw;
-            const unsigned grid = (unsigned)(waves > 0 ? (waves + 3) / 4 : 1);
-            hipLaunchKernelGGL((laswp_panel_kernel<T, NB>), dim3(grid), dim3(256), 0, s, A, lda,
-                               n, j0, j0 + jb, n, j0, jb, piv, perm, cpw);
+    // Interchanges of panel [j0, j0+jb) on columns [a0,a1) u [b0,b1).
+    auto swap_cols = [&](i64 j0, int jb, i64 a0, i64 a1, i64 b0, i64 b1, i64* pm) -> int {
+        const i64 cols = (a1 - a0) + (b1 - b0);
+        if (cols <= 0 && !pm) return TMVX_OK;
+        const int cpw = 4;
+        const i64 waves = (cols + cpw - 1) / cpw;
+        const unsigned grid = (unsigned)(waves > 0 ? (waves + 3) / 4 : 1);
+        hipLaunchKernelGGL((laswp_panel_kernel<T, NB>), dim3(grid), dim3(256), 0, s, A, lda, n,
+                           a0, a1, b0, b1, j0, jb, piv, pm, cpw);
+        return hip_status(hipGetLastError());
+    };
+    // Rows [j0, j0+jb) of columns [c0, c0+nc) := L(j0..)^-1 times themselves.
+    auto solve_rows = [&](i64 j0, int jb, i64 c0, i64 nc) -> int {
+        if (nc <= 0) return TMVX_OK;
+        const int cpw = 16;
+        const i64 waves = (nc + cpw - 1) / cpw;
+        hipLaunchKernelGGL((trsm_u12_kernel<T, NB>), dim3((unsigned)((waves + 3) / 4)),
+                           dim3(256), 0, s, A, lda, j0, jb, c0, nc, cpw);
+        return hip_status(hipGetLastError());
+    };
+    // Two-level blocking: outer panels of NBO columns are factorised as
+    // NB-column panels updated only inside the outer panel; the trailing
+    // matrix then takes one update with K = NBO (4x the arithmetic per
+    // byte of C moved compared with K = NB).
+    constexpr i64 NBO = 4 * NB;
+    for (i64 J0 = 0; J0 < n && rc == TMVX_OK; J0 += NBO) {
+        const i64 JE = (n - J0) < NBO ? n : J0 + NBO;      // outer panel [J0, JE)
+        for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
+            const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
+            const int G = (int)((n - j0 + NTH - 1) / NTH);
+            hipLaunchKernelGGL((panel_kernel<T, NB, NTH>), dim3(G), dim3(NTH), 0, s,
+                               A, lda, n, j0, jb, piv, wk);
             if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
+            // inside the outer panel only; the rest is deferred
+            if ((rc = swap_cols(j0, jb, J0, j0, j0 + jb, JE, perm)) != TMVX_OK) break;
+            const i64 nr = JE - j0 - jb;
+            if (nr > 0) {
+                if ((rc = solve_rows(j0, jb, j0 + jb, nr)) != TMVX_OK) break;
+                rc = gemm_launch<T>(0, 0, n - j0 - jb, nr, jb, T(-1), A + (j0 + jb) + j0 * lda,
+                                    lda, A + j0 + (j0 + jb) * lda, lda, T(1),
+                                    A + (j0 + jb) + (j0 + jb) * lda, lda, s);
+            }
         }
-        const i64 nr = n - j0 - jb;
-        if (nr > 0) {
-            const int cpw = 16;
-            const i64 waves = (nr + cpw - 1) / cpw;
-            hipLaunchKernelGGL((trsm_u12_kernel<T, NB>), dim3((unsigned)((waves + 3) / 4)),
-                               dim3(256), 0, s, A, lda, j0, jb, nr, cpw);
-            if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
-            rc = gemm_launch<T>(0, 0, nr, nr, jb, T(-1), A + (j0 + jb) + j0 * lda, lda,
-                                A + j0 + (j0 + jb) * lda, lda, T(1),
-                                A + (j0 + jb) + (j0 + jb) * lda, lda, s);
+        if (rc != TMVX_OK) break;
+        // deferred interchanges on the columns outside the outer panel
+        for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
+            const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
+            rc = swap_cols(j0, jb, 0, J0, JE, n, nullptr);
         }
+        const i64 nr = n - JE;
+        if (nr <= 0 || rc != TMVX_OK) continue;
+        // U12 = L11^-1 A12 for the outer block row, NB rows at a time
+        for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
+            const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
+            if ((rc = solve_rows(j0, jb, JE, nr)) != TMVX_OK) break;
+            const i64 below = JE - j0 - jb;
+            if (below > 0)
+                rc = gemm_launch<T>(0, 0, below, nr, jb, T(-1), A + (j0 + jb) + j0 * lda, lda,
+                                    A + j0 + JE * lda, lda, T(1), A + (j0 + jb) + JE * lda, lda,
+                                    s);
+        }
+        if (rc != TMVX_OK) break;
+        // A22 -= A21 * U12 with K = JE - J0
+        rc = gemm_launch<T>(0, 0, nr, nr, JE - J0, T(-1), A + JE + J0 * lda, lda,
+                            A + J0 + JE * lda, lda, T(1), A + JE + JE * lda, lda, s);
     }
     int herr = 0;
     i64 fz = n + 1;

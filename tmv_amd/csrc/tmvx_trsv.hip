@@ -74,11 +74,18 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
         if (in_perm) { src = in_perm[b0 + lane]; if (src < 0 || src >= n) src = b0 + lane; }
         acc = Bin[src + wave * ldb];
     }
-    // Contributions of the finished blocks, in solve order.
+    // Contributions of the finished blocks, in solve order.  The A block
+    // (64 values per lane) is loaded before waiting for the block's flag,
+    // so its latency overlaps the wait.
     for (i64 qc = 0; qc < q; ++qc) {
         const i64 cb = lower ? qc : nblk - 1 - qc;
         const i64 c0 = cb * NB;
         const int cs = (int)((n - c0) < NB ? (n - c0) : NB);
+        T ablk[NB];
+        static_for<0, NB>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            ablk[k] = (has && lane < bs && k < cs) ? opA(b0 + lane, c0 + k) : T(0);
+        });
         if (t == 0) {
             unsigned spins = 0;
             while (__hip_atomic_load(wk.flags + cb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
@@ -95,12 +102,16 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
         }
         __syncthreads();
         if (s_fail) return;
-        if (has && lane < cs) s_x[wave][lane] = sv_cload(xpub + c0 + lane + (i64)wave * n);
+        if (has) s_x[wave][lane] = lane < cs ? sv_cload(xpub + c0 + lane + (i64)wave * n) : T(0);
         __syncthreads();
         if (has && lane < bs) {
-            T s = T(0);
-            for (int k = 0; k < cs; ++k) s += opA(b0 + lane, c0 + k) * s_x[wave][k];
-            acc -= s;
+            T s0 = T(0), s1 = T(0);
+            static_for<0, NB / 2>([&](auto kc) {
+                constexpr int k = 2 * decltype(kc)::value;
+                s0 += ablk[k] * s_x[wave][k];
+                s1 += ablk[k + 1] * s_x[wave][k + 1];
+            });
+            acc -= s0 + s1;
         }
         __syncthreads();
     }
