@@ -1,0 +1,229 @@
+// TMV-style tests of the drop-in C++ layer (include/tmvx_tmv.h) on the GPU.
+// They follow TMV's own tests and examples (tests/TMV_TestMatrixDiv.cpp,
+// tests/TMV_TestMatrixDet.cpp, examples/Division.cpp, examples/Matrix.cpp):
+// the same matrices, the same division syntax, the same expected values.
+// Exit status 0 when every check passes.
+#include "tmvx_tmv.h"
+
+#include <cmath>
+#include <cstdio>
+
+static int nfail = 0;
+#define Assert(x, s)                                                           \
+    do {                                                                       \
+        if (!(x)) { std::printf("FAILED: %s (%s:%d)\n", s, __FILE__, __LINE__); ++nfail; } \
+        else std::printf("ok: %s\n", s);                                       \
+    } while (0)
+
+static const double EPS = 10 * std::numeric_limits<double>::epsilon();
+
+template <class T, int S>
+static double NormF(const tmv::Matrix<T, S>& m)
+{
+    double s = 0;
+    for (ptrdiff_t j = 0; j < m.rowsize(); ++j)
+        for (ptrdiff_t i = 0; i < m.colsize(); ++i) s += double(m(i, j)) * double(m(i, j));
+    return std::sqrt(s);
+}
+
+template <class T>
+static double Norm(const tmv::Vector<T>& v)
+{
+    double s = 0;
+    for (ptrdiff_t i = 0; i < v.size(); ++i) s += double(v(i)) * double(v(i));
+    return std::sqrt(s);
+}
+
+template <class T, int S>
+static tmv::Vector<T> MatVec(const tmv::Matrix<T, S>& m, const tmv::Vector<T>& x)
+{
+    tmv::Vector<T> y(m.colsize());
+    for (ptrdiff_t i = 0; i < m.colsize(); ++i) {
+        long double s = 0;
+        for (ptrdiff_t j = 0; j < m.rowsize(); ++j) s += (long double)m(i, j) * x(j);
+        y(i) = (T)s;
+    }
+    return y;
+}
+
+template <class T, int S>
+static tmv::Vector<T> VecMat(const tmv::Vector<T>& x, const tmv::Matrix<T, S>& m)
+{
+    tmv::Vector<T> y(m.rowsize());
+    for (ptrdiff_t j = 0; j < m.rowsize(); ++j) {
+        long double s = 0;
+        for (ptrdiff_t i = 0; i < m.colsize(); ++i) s += (long double)x(i) * m(i, j);
+        y(j) = (T)s;
+    }
+    return y;
+}
+
+// tests/TMV_TestMatrixDiv.cpp:8-88, for both storage orders.
+template <int S>
+static void TestSquareDiv()
+{
+    tmv::Matrix<double, S> m(4, 4);
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) m(i, j) = 2 + 4 * i - 5 * j;
+    m(0, 0) = 14; m(1, 0) = -2; m(2, 0) = 7; m(3, 0) = -10; m(2, 2) = 30;
+    tmv::Vector<double> b{2, -10, 5, -5};
+    m.divideUsing(tmv::LU);
+    m.saveDiv();
+    m.setDiv();
+    Assert(m.lud().isTrans() == (S == tmv::RowMajor), "isTrans follows the storage order");
+    tmv::Vector<double> x = b / m;
+    tmv::Vector<double> b2 = MatVec(m, x);
+    tmv::Vector<double> d(4);
+    for (int i = 0; i < 4; ++i) d(i) = b2(i) - b(i);
+    Assert(Norm(d) < 1e3 * EPS * Norm(b), "Square b/m");
+    x = b % m;
+    b2 = VecMat(x, m);
+    for (int i = 0; i < 4; ++i) d(i) = b2(i) - b(i);
+    Assert(Norm(d) < 1e3 * EPS * Norm(b), "Square b%m");
+    const double det = m.det();
+    Assert(std::fabs(det - 28800) < 1e-9 * 28800, "Square Det = 28800");
+    double sg;
+    Assert(std::fabs(m.lud().logDet(&sg) - std::log(28800.0)) < 1e-12 && sg == 1.0,
+           "Square logDet and sign");
+}
+
+// examples/Division.cpp:8-73, including the saveDiv behaviour.
+static void TestDivisionExample()
+{
+    tmv::Matrix<double> A(4, 4);
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) A(i, j) = 6. * i - 2 * j * j + 2.;
+    for (int i = 0; i < 4; ++i) A(i, i) += 5.;
+    tmv::Vector<double> b(4);
+    for (int i = 0; i < 4; ++i) b(i) = 3. + 2. * i;
+    auto close6 = [](const tmv::Vector<double>& x, std::initializer_list<double> g) {
+        int i = 0;
+        for (double v : g) { if (std::fabs(x(i) - v) > 5e-6 * std::fabs(v) + 1e-12) return false; ++i; }
+        return true;
+    };
+    tmv::Vector<double> x = b / A;
+    Assert(close6(x, {0.294545, 0.170909, 0.0472727, -0.0763636}), "Division: x = b/A");
+    A(0, 0) = 20.;
+    x = b / A;
+    Assert(close6(x, {0.133458, 0.29877, 0.117091, -0.064587}), "Division: re-solve after change");
+    tmv::Matrix<double> A2 = A;
+    A2.saveDiv();
+    x = b / A2;
+    Assert(close6(x, {0.133458, 0.29877, 0.117091, -0.064587}), "Division: saveDiv x1");
+    tmv::Vector<double> br{9, 7, 5, 3};
+    x = br / A2;
+    Assert(close6(x, {0.136753, 0.207381, -0.0775483, -0.362478}), "Division: reuse factors x2");
+    for (int j = 0; j < 4; ++j) A2(0, j) *= 2.;
+    x = b / A2;   // stale factorisation, as in TMV
+    Assert(close6(x, {0.133458, 0.29877, 0.117091, -0.064587}), "Division: saveDiv keeps old factors");
+    A2.resetDiv();
+    x = b / A2;
+    Assert(close6(x, {0.0703537, 0.348858, 0.144442, -0.0599736}), "Division: resetDiv refactorises");
+}
+
+// examples/Matrix.cpp:297-334 in every storage combination.
+template <int SA, int SB>
+static void TestProduct()
+{
+    const double m1v[12] = {42, 32, 33, 21, 26, 12, 33, 11, 8, 29, 27, 20};
+    const double m5v[9] = {0, 1, 2, 2, 0, -2, 4, -1, -6};
+    tmv::Matrix<double, SA> m1(4, 3);
+    tmv::Matrix<double, SB> m5(3, 3);
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 3; ++j) m1(i, j) = m1v[i * 3 + j];
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) m5(i, j) = m5v[i * 3 + j];
+    tmv::Matrix<double> p = m1 * m5;
+    const double g[12] = {196, 9, -178, 100, 9, -82, 54, 25, -4, 134, 9, -116};
+    bool ok = true;
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 3; ++j) ok = ok && p(i, j) == g[i * 3 + j];
+    Assert(ok, "m1 * m5 (exact)");
+    tmv::Matrix<double, tmv::RowMajor> c(4, 3, 1.0);
+    tmv::MultMM<true>(-2.0, m1, m5, c);     // c += -2 m1 m5 into row-major C
+    ok = true;
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 3; ++j) ok = ok && c(i, j) == 1 - 2 * g[i * 3 + j];
+    Assert(ok, "MultMM<true> into row-major C");
+}
+
+// tests/TMV_TestMatrixDet.cpp exact determinants.
+static void TestDet()
+{
+    tmv::Matrix<double> m4(4, 4, {6, 1, 2, 4, 3, 2, 1, 8, 5, 6, 7, 1, 2, 9, 8, 3});
+    Assert(std::fabs(m4.det() + 66) < 66 * 1e3 * EPS, "4x4 determinant -66");
+    tmv::Matrix<double> m5(5, 5, {6, 1, 2, 4, 1, 3, 2, 1, 8, 4, 5, 6, 7, 1, 2, 2, 9, 8, 3, 7, 1, 4, 5, 9, 3});
+    Assert(std::fabs(m5.det() + 1118) < 1118 * 1e3 * EPS, "5x5 determinant -1118");
+}
+
+// LU_Decompose + getP/getL/getU reconstruct PA = LU (TMV_TestMatrixDecomp).
+static void TestDecomp()
+{
+    const int N = 200;
+    tmv::Matrix<double> m(N, N);
+    for (int i = 0; i < N; ++i) for (int j = 0; j < N; ++j) m(i, j) = (2.0 + 4 * i - 5 * j) / (10 * N);
+    m(3, 3) = 14; m(3, 4) = -2; m(0, 2) = 7; m(N - 1, N - 4) = 23; m(N - 1, N - 2) = 13; m(N - 1, N - 1) = -10;
+    for (int i = 0; i < N; ++i) m(i, i) *= 30 * N;
+    m.divideUsing(tmv::LU);
+    m.saveDiv();
+    const tmv::LUDiv<double>& lu = m.lud();
+    tmv::Matrix<double> L = lu.getL(), U = lu.getU();
+    tmv::Matrix<double> LUp = L * U;
+    tmv::Matrix<double> PA = m;
+    for (int i = 0; i < N; ++i) {
+        const ptrdiff_t p = lu.getP()[i];
+        if (p != i) for (int j = 0; j < N; ++j) std::swap(PA(i, j), PA(p, j));
+    }
+    double e = 0;
+    for (int j = 0; j < N; ++j) for (int i = 0; i < N; ++i) e = std::fmax(e, std::fabs(PA(i, j) - LUp(i, j)));
+    double amax = 0;
+    for (int j = 0; j < N; ++j) for (int i = 0; i < N; ++i) amax = std::fmax(amax, std::fabs(m(i, j)));
+    Assert(e <= 8 * N * std::numeric_limits<double>::epsilon() * amax, "P A = L U to 8 n eps");
+    Assert(!lu.isSingular(), "regular fixture is not singular");
+    tmv::Matrix<double> A2 = m;
+    tmv::Permutation P;
+    tmv::LU_Decompose(A2, P);
+    bool same = true;
+    for (int i = 0; i < N; ++i) same = same && P[i] == lu.getP()[i];
+    Assert(same, "LU_Decompose(A, P) gives the divider's pivots");
+}
+
+static void TestSingular()
+{
+    tmv::Matrix<double> m(20, 20, 0.0);
+    for (int i = 0; i < 20; ++i) for (int j = 0; j < 20; ++j) m(i, j) = 2.0 + 4 * i - 5 * j;
+    for (int i = 0; i < 20; ++i) m(i, 1) = 0;
+    tmv::Vector<double> b(20, 1.0);
+    bool thrown = false;
+    try { tmv::Vector<double> x = b / m; (void)x; }
+    catch (tmv::Singular&) { thrown = true; }
+    Assert(thrown, "b/m on a singular matrix throws tmv::Singular");
+}
+
+static void TestMixed()
+{
+    const int N = 300;
+    tmv::Matrix<double> m(N, N);
+    for (int i = 0; i < N; ++i) for (int j = 0; j < N; ++j) m(i, j) = std::sin(1.0 + i * 0.37 + j * j * 0.011);
+    for (int i = 0; i < N; ++i) m(i, i) += N;
+    tmv::Vector<double> b(N);
+    for (int i = 0; i < N; ++i) b(i) = std::cos(0.1 * i);
+    tmvx_ir_info info;
+    tmv::Vector<double> x = tmv::SolveMixed(b, m, 1e-12, &info);
+    tmv::Vector<double> x64 = b / m;
+    double d = 0, xm = 0;
+    for (int i = 0; i < N; ++i) { d = std::fmax(d, std::fabs(x(i) - x64(i))); xm = std::fmax(xm, std::fabs(x64(i))); }
+    Assert(info.converged && info.backward_error <= 1e-12 && d <= 1e-12 * xm,
+           "SolveMixed agrees with b/m");
+}
+
+int main()
+{
+    TestSquareDiv<tmv::ColMajor>();
+    TestSquareDiv<tmv::RowMajor>();
+    TestDivisionExample();
+    TestProduct<tmv::ColMajor, tmv::ColMajor>();
+    TestProduct<tmv::ColMajor, tmv::RowMajor>();
+    TestProduct<tmv::RowMajor, tmv::ColMajor>();
+    TestProduct<tmv::RowMajor, tmv::RowMajor>();
+    TestDet();
+    TestDecomp();
+    TestSingular();
+    TestMixed();
+    std::printf("%s: %d failure(s)\n", nfail ? "FAILED" : "PASSED", nfail);
+    return nfail ? 1 : 0;
+}

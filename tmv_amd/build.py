@@ -62,6 +62,23 @@ def _gen_gemm_instances() -> list[str]:
     return out
 
 
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_tmv_shim.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_tmv_shim")
+
+
+def build_cpp_tests() -> str:
+    """The C++ program testing the TMV-shaped layer (include/tmvx_tmv.h)
+    against libtmvx.so; run by tests/test_gpu_shim.py."""
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC,
+           "-o", CPP_TEST_BIN + ".tmp", "-L", HERE, "-ltmvx",
+           "-Wl,-rpath,$ORIGIN/../../tmv_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ test build failed:\n{r.stderr[-6000:]}")
+    os.replace(CPP_TEST_BIN + ".tmp", CPP_TEST_BIN)
+    return CPP_TEST_BIN
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip"))) + _gen_gemm_instances()
@@ -73,8 +90,9 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
+    build_cpp_tests()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CPP_TEST_BIN}")
     return LIB
 
 

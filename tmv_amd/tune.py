@@ -33,7 +33,7 @@ def _bind():
 
 
 def sweep(n=8192, dtypes=("f64", "f32"), reps=3, layouts=("CC", "CR", "RC", "RR"),
-          m=None, k=None):
+          m=None, k=None, cfgs=None):
     L = _bind()
     m = m or n
     k = k or n
@@ -49,7 +49,7 @@ def sweep(n=8192, dtypes=("f64", "f32"), reps=3, layouts=("CC", "CR", "RC", "RR"
         A, B = A.to(dt), B.to(dt)
         C = torch.empty(m * n, dtype=dt, device="cuda")
         fn = getattr(L, "tmvx_gemm_cfg_" + suf)
-        for cfg in range(L.tmvx_gemm_config_count(eb)):
+        for cfg in (cfgs if cfgs is not None else range(L.tmvx_gemm_config_count(eb))):
             name = L.tmvx_gemm_config_name(eb, cfg).decode()
             for lay in layouts:
                 la = 0 if lay[0] == "C" else 1
@@ -88,9 +88,23 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--layouts", default="CC,CR,RC,RR")
     ap.add_argument("--json", default=None)
+    ap.add_argument("--cfg", type=int, action="append", default=None,
+                    help="only these configuration ids")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="instead: time a plain device-to-device copy of this size "
+                         "(counter calibration)")
     a = ap.parse_args()
+    if a.copy_mib:
+        nb = a.copy_mib * (1 << 20)
+        src = torch.empty(nb // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(a.reps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copied {a.copy_mib} MiB x {a.reps}")
+        return
     dts = ("f64", "f32") if a.dtype == "both" else (a.dtype,)
-    res = sweep(a.n, dts, a.reps, tuple(a.layouts.split(",")), a.m, a.k)
+    res = sweep(a.n, dts, a.reps, tuple(a.layouts.split(",")), a.m, a.k, a.cfg)
     if a.json:
         with open(a.json, "w") as f:
             json.dump(res, f, indent=1)
