@@ -37,6 +37,10 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 SEED = 0x7A15EED
+# HBM bytes per launch of the 8192^3 CCC DGEMM (gemm_kernel f64 128x128x16),
+# from rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the factors a
+# 1 GiB device copy reads on the same counters; profiles/dgemm_8192_ccc_traffic.md.
+DGEMM_8192_CCC_TRAFFIC = 13.71e9
 
 
 def conditioned_gpu(n, kappa, seed, torch, tmv_amd):
@@ -225,7 +229,8 @@ def main():
         "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
         "dgemm_8192": gemm,
         "roofline": {"bound": "mfma", "achieved": ccc["tflops"], "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": ccc["frac_peak"], "traffic": None,
+                     "unit": "TFLOP/s", "frac": ccc["frac_peak"],
+                     "traffic": DGEMM_8192_CCC_TRAFFIC if gn == 8192 else None,
                      "kernel": "gemm_kernel f64 128x128x16 (DGEMM 8192^3, CCC)",
                      "flops_per_launch": 2 * gn ** 3},
         "cpu_baseline": cpu,
