@@ -53,6 +53,18 @@ _SIGS = {
     "tmvx_fill_uniform_f64": (_int, [ctypes.c_uint64, ctypes.c_uint64, _i64, _d, _d, _vp, _vp]),
     "tmvx_fill_normal_f64": (_int, [ctypes.c_uint64, ctypes.c_uint64, _i64, _vp, _vp]),
     "tmvx_gesv_mixed_f64": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _d, _int, _vp, _vp]),
+    # batched systems (include/tmvx_batched.h)
+    "tmvx_batch_plan_create": (_int, [_int, _i64, _vp, _vp, _int, ctypes.POINTER(_vp), _vp]),
+    "tmvx_batch_plan_destroy": (_int, [_vp]),
+    "tmvx_batch_plan_describe": (ctypes.c_char_p, [_vp]),
+    "tmvx_batch_plan_set": (_int, [_vp, _int, _int, _int, _vp]),
+    "tmvx_batch_schedule_count": (_int, []),
+    "tmvx_gesv_batched_f64": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "tmvx_gesv_batched_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "tmvx_getrs_batched_f64": (_int, [_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "tmvx_getrs_batched_f32": (_int, [_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "tmvx_gemm_batched_f64": (_int, [_i64, _vp, _vp, _vp, _d, _vp, _vp, _d, _vp, _vp]),
+    "tmvx_gemm_batched_f32": (_int, [_i64, _vp, _vp, _vp, _f, _vp, _vp, _f, _vp, _vp]),
     # tuning hooks (include/tmvx_tune.h)
     "tmvx_gemm_config_count": (_int, [_int]),
     "tmvx_gemm_config_name": (ctypes.c_char_p, [_int, _int]),
@@ -290,3 +302,130 @@ def fill_normal(out, seed, offset=0, stream=None):
     _check(lib().tmvx_fill_normal_f64(seed, offset, out.numel(), out.data_ptr(),
                                       _stream(stream)), "fill_normal")
     return out
+
+
+# ---------------------------------------------------------------------------
+# Batches of small systems (include/tmvx_batched.h)
+# ---------------------------------------------------------------------------
+
+BATCH_NMAX = 128
+BATCH_RHSMAX = 16
+
+
+class Batch:
+    """A batch of independent systems packed in one device buffer each for
+    A (n_i x n_i), B (n_i x nrhs_i), the pivots and info, column-major with
+    leading dimension n_i, plus the device pointer arrays the C ABI takes.
+    ``a(i)`` / ``b(i)`` / ``piv(i)`` are views of system i."""
+
+    def __init__(self, ns, nrhs=None, dtype=None, device="cuda"):
+        torch = _torch()
+        import numpy as np
+        dtype = torch.float64 if dtype is None else dtype
+        self.dtype = dtype
+        self.ns = np.asarray(ns, dtype=np.int32)
+        self.nrhs = (np.zeros_like(self.ns) if nrhs is None
+                     else np.asarray(nrhs, dtype=np.int32))
+        if self.nrhs.shape != self.ns.shape:
+            raise ValueError("ns and nrhs differ in length")
+        cnt = self.count = len(self.ns)
+        na = self.ns.astype(np.int64) ** 2
+        nb = self.ns.astype(np.int64) * self.nrhs
+        self.offA = np.concatenate([[0], np.cumsum(na)[:-1]]).astype(np.int64) if cnt else na
+        self.offB = np.concatenate([[0], np.cumsum(nb)[:-1]]).astype(np.int64) if cnt else nb
+        self.offP = (np.concatenate([[0], np.cumsum(self.ns)[:-1]]).astype(np.int64)
+                     if cnt else self.ns.astype(np.int64))
+        self.A = torch.zeros(max(int(na.sum()), 1), dtype=dtype, device=device)
+        self.B = torch.zeros(max(int(nb.sum()), 1), dtype=dtype, device=device)
+        self.P = torch.zeros(max(int(self.ns.sum()), 1), dtype=torch.int64, device=device)
+        self.info = torch.zeros(max(cnt, 1), dtype=torch.int32, device=device)
+        es = self.A.element_size()
+        self.d_n = torch.as_tensor(self.ns, device=device)
+        self.d_nrhs = torch.as_tensor(self.nrhs, device=device)
+        self.pA = torch.as_tensor(self.A.data_ptr() + es * self.offA, device=device)
+        self.pB = torch.as_tensor(self.B.data_ptr() + es * self.offB, device=device)
+        self.pP = torch.as_tensor(self.P.data_ptr() + 8 * self.offP, device=device)
+
+    def a(self, i):
+        n = int(self.ns[i]); o = int(self.offA[i])
+        return self.A[o:o + n * n].view(n, n).t()
+
+    def b(self, i):
+        n, r = int(self.ns[i]), int(self.nrhs[i]); o = int(self.offB[i])
+        return self.B[o:o + n * r].view(r, n).t()
+
+    def piv(self, i):
+        n = int(self.ns[i]); o = int(self.offP[i])
+        return self.P[o:o + n]
+
+
+class BatchPlan:
+    """tmvx_batch_plan: the work list and schedule of one batch shape."""
+
+    def __init__(self, batch_or_ns, nrhs=None, dtype=None, tune=False, stream=None):
+        torch = _torch()
+        import numpy as np
+        if isinstance(batch_or_ns, Batch):
+            ns, nrhs, dtype = batch_or_ns.ns, batch_or_ns.nrhs, batch_or_ns.dtype
+        else:
+            ns = np.asarray(batch_or_ns, dtype=np.int32)
+            nrhs = np.zeros_like(ns) if nrhs is None else np.asarray(nrhs, dtype=np.int32)
+        dtype = torch.float64 if dtype is None else dtype
+        self.dtype = dtype
+        self._ns = np.ascontiguousarray(ns, dtype=np.int32)
+        self._nrhs = np.ascontiguousarray(nrhs, dtype=np.int32)
+        h = _vp()
+        rc = lib().tmvx_batch_plan_create(0 if dtype == torch.float64 else 1, len(self._ns),
+                                          self._ns.ctypes.data, self._nrhs.ctypes.data,
+                                          int(bool(tune)), ctypes.byref(h), _stream(stream))
+        _check(rc, "batch_plan_create")
+        self.handle = h
+
+    def describe(self) -> str:
+        return lib().tmvx_batch_plan_describe(self.handle).decode()
+
+    def set(self, schedule, split, wgs_per_cu, stream=None):
+        _check(lib().tmvx_batch_plan_set(self.handle, schedule, split, wgs_per_cu,
+                                         _stream(stream)), "batch_plan_set")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().tmvx_batch_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_schedule_count() -> int:
+    return lib().tmvx_batch_schedule_count()
+
+
+def gesv_batched(plan: BatchPlan, batch: Batch, stream=None):
+    """Factor every A_i in place and solve A_i X_i = B_i (B overwritten), in
+    one launch.  Returns the per-system info tensor (device)."""
+    fn = getattr(lib(), "tmvx_gesv_batched_" + _suffix(batch.dtype))
+    _check(fn(plan.handle, batch.pA.data_ptr(), batch.pP.data_ptr(), batch.pB.data_ptr(),
+              batch.info.data_ptr(), _stream(stream)), "gesv_batched")
+    return batch.info
+
+
+def getrs_batched(batch: Batch, trans=False, stream=None):
+    """Solve with the stored factors of ``batch`` (B overwritten)."""
+    fn = getattr(lib(), "tmvx_getrs_batched_" + _suffix(batch.dtype))
+    _check(fn(int(bool(trans)), batch.count, batch.d_n.data_ptr(), batch.d_nrhs.data_ptr(),
+              batch.pA.data_ptr(), batch.pP.data_ptr(), batch.pB.data_ptr(),
+              _stream(stream)), "getrs_batched")
+
+
+def gemm_batched(m, n, k, pA, pB, pC, alpha=1.0, beta=0.0, dtype=None, stream=None):
+    """C_i = alpha A_i B_i + beta C_i over device size arrays m, n, k (int32)
+    and device pointer arrays pA, pB, pC (int64)."""
+    torch = _torch()
+    dtype = torch.float64 if dtype is None else dtype
+    fn = getattr(lib(), "tmvx_gemm_batched_" + _suffix(dtype))
+    _check(fn(m.numel(), m.data_ptr(), n.data_ptr(), k.data_ptr(), alpha, pA.data_ptr(),
+              pB.data_ptr(), beta, pC.data_ptr(), _stream(stream)), "gemm_batched")
