@@ -17,6 +17,10 @@ Also measured in the same run (outside the K timed steps):
   - DGEMM M = N = K = 8192 in the CCC, CRC and RCC operand layouts; the
     CCC kernel is the `roofline` entry (FP64 MFMA, 78.6 TFLOP/s peak),
     timed with HIP events on the stream it runs on;
+  - the batched workload: 20,000 systems n ~ U{8..128} (+ n on the
+    diagonal), nrhs ~ U{1..4}, FP64 and FP32, factor + solve in one
+    hipGraph-replayed persistent launch (autotuned plan) vs per-matrix
+    launches, plus the batched GEMM with K = n;
   - the CPU baseline: the real TMV 0.76 library (oracle/_ref, built from
     the reference sources) on a bounded sample, x = b/A at n = 4096 with
     OpenMP on the host's cores (kind "reference"); the oracle restatement
@@ -100,6 +104,123 @@ def cpu_baseline(seconds_cap=30.0):
             "ms": round(best * 1e3, 1)}
 
 
+def batched_workload(torch, tmv_amd, dtype, count=20000, reps=5, per_matrix_sample=200):
+    """20,000 independent systems, n ~ U{8..128}, entries uniform[-1,1] + n on
+    the diagonal, nrhs ~ U{1..4}: factor + solve in one persistent launch
+    replayed from a hipGraph (autotuned plan), the batched GEMM C_i = A_i B_i
+    with K = n_i, and the same systems through per-matrix launches."""
+    import numpy as np
+    g = np.random.default_rng(SEED + 30)
+    ns = g.integers(8, 129, count).astype(np.int32)
+    nrhs = g.integers(1, 5, count).astype(np.int32)
+    b = tmv_amd.Batch(ns, nrhs, dtype=dtype)
+    tmp = torch.empty(b.A.numel(), dtype=torch.float64, device="cuda")
+    tmv_amd.fill_uniform(tmp, SEED + 31)
+    diag = np.concatenate([b.offA[i] + np.arange(n) * (n + 1) for i, n in enumerate(ns)])
+    dval = np.repeat(ns.astype(np.float64), ns)
+    tmp[torch.as_tensor(diag, device="cuda")] += torch.as_tensor(dval, device="cuda")
+    A0 = tmp.to(dtype)
+    tmp = torch.empty(b.B.numel(), dtype=torch.float64, device="cuda")
+    tmv_amd.fill_uniform(tmp, SEED + 32)
+    B0 = tmp.to(dtype)
+    del tmp
+    t0 = time.perf_counter()
+    plan = tmv_amd.BatchPlan(b, tune=True)
+    tune_s = time.perf_counter() - t0
+    s = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    b.A.copy_(A0); b.B.copy_(B0)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph, stream=s):
+        tmv_amd.gesv_batched(plan, b, stream=s)
+    times = []
+    for _ in range(reps + 1):
+        b.A.copy_(A0); b.B.copy_(B0)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(s):
+            e0.record(s); graph.replay(); e1.record(s)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = min(times[1:])
+    bad = int((b.info != 0).sum().item())
+    # solution check on a sample: backward error per column
+    worst = 0.0
+    for i in range(0, count, max(1, count // 50)):
+        n = int(ns[i])
+        o = int(b.offA[i])
+        Ai = A0[o:o + n * n].view(n, n).t().double()
+        X = b.b(i).double()
+        ob = int(b.offB[i])
+        Bi = B0[ob:ob + n * int(nrhs[i])].view(int(nrhs[i]), n).t().double()
+        r = Bi - Ai @ X
+        be = (r.abs().max(dim=0).values /
+              (Ai.abs().sum(dim=1).max() * X.abs().max(dim=0).values + Bi.abs().max(dim=0).values))
+        worst = max(worst, float(be.max()))
+    n64 = ns.astype(np.float64)
+    flops = float((2.0 / 3.0 * n64 ** 3 + 2.0 * n64 ** 2 * nrhs).sum())
+    esz = b.A.element_size()
+    bytes_ = float((2 * n64 ** 2 + 2 * n64 * nrhs).sum() * esz + n64.sum() * 8)
+    out = {"systems": count, "dtype": str(dtype).replace("torch.", ""), "ms": round(ms, 3),
+           "tflops": round(flops / ms / 1e9, 3), "gbs": round(bytes_ / ms / 1e6, 1),
+           "hbm_frac": round(bytes_ / ms / 1e6 / 8000.0, 4),
+           "singular": bad, "worst_backward_error": worst,
+           "plan": plan.describe(), "tune_s": round(tune_s, 2),
+           "algorithmic": {"flops": flops, "bytes": bytes_}}
+    # per-matrix launches: the same kernel launched once per system (sample)
+    k = min(per_matrix_sample * 10, count)
+    plans = [tmv_amd.BatchPlan([int(ns[i])], [int(nrhs[i])], dtype=dtype) for i in range(k)]
+    singles = [tmv_amd.Batch([int(ns[i])], [int(nrhs[i])], dtype=dtype) for i in range(k)]
+    for i in range(k):
+        singles[i].A.copy_(A0[int(b.offA[i]):int(b.offA[i]) + int(ns[i]) ** 2])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(k):
+        tmv_amd.gesv_batched(plans[i], singles[i])
+    torch.cuda.synchronize()
+    per_launch = (time.perf_counter() - t1) / k
+    del plans, singles
+    # per-matrix general path (blocked getrf + getrs, as a TMV caller's loop)
+    k2 = min(per_matrix_sample, count)
+    Ms = [A0[int(b.offA[i]):int(b.offA[i]) + int(ns[i]) ** 2].view(int(ns[i]), int(ns[i])).t().clone()
+          for i in range(k2)]
+    Ms = [tmv_amd.colmajor(m) for m in Ms]
+    Xs = [torch.ones(int(ns[i]), int(nrhs[i]), dtype=dtype, device="cuda").t().contiguous().t()
+          for i in range(k2)]
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for i in range(k2):
+        piv, _ = tmv_amd.getrf(Ms[i])
+        tmv_amd.getrs(Ms[i], piv, Xs[i], check_singular=False)
+    torch.cuda.synchronize()
+    per_general = (time.perf_counter() - t2) / k2
+    out["per_matrix_launch_ms"] = round(per_launch * count * 1e3, 2)
+    out["per_matrix_launch_sample"] = k
+    out["per_matrix_getrf_getrs_ms"] = round(per_general * count * 1e3, 1)
+    out["per_matrix_getrf_getrs_sample"] = k2
+    out["speedup_vs_per_matrix_launch"] = round(per_launch * count * 1e3 / ms, 1)
+    # batched GEMM, K = n: C_i = A_i B_i (A_i and B_i the systems' matrices)
+    C = torch.empty_like(b.A)
+    pC = torch.as_tensor(C.data_ptr() + esz * b.offA, device="cuda")
+    pA0 = torch.as_tensor(A0.data_ptr() + esz * b.offA, device="cuda")
+    gt = []
+    for _ in range(reps + 1):
+        e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        tmv_amd.gemm_batched(b.d_n, b.d_n, b.d_n, pA0, pA0, pC, dtype=dtype)
+        e1.record()
+        torch.cuda.synchronize()
+        gt.append(e0.elapsed_time(e1))
+    gms = min(gt[1:])
+    gflops = float((2.0 * n64 ** 3).sum())
+    gbytes = float((3 * n64 ** 2).sum() * esz)
+    out["gemm"] = {"ms": round(gms, 3), "tflops": round(gflops / gms / 1e9, 2),
+                   "gbs": round(gbytes / gms / 1e6, 1)}
+    del b, A0, B0, C, graph
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +233,8 @@ def main():
     ap.add_argument("--gemm-n", type=int, default=8192)
     ap.add_argument("--gemm-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--only-batched", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
@@ -122,6 +245,11 @@ def main():
     import tmv_amd
     tmv_amd.lib()                      # the HIP library; no fallback
     n, nrhs = a.n, a.nrhs
+    if a.only_batched:                 # the batched workload alone (profiling)
+        print(json.dumps({"batched_20000": {
+            "f64": batched_workload(torch, tmv_amd, torch.float64),
+            "f32": batched_workload(torch, tmv_amd, torch.float32)}}), flush=True)
+        return
 
     # ---- inputs (untimed) -------------------------------------------------
     t_setup = time.perf_counter()
@@ -194,6 +322,11 @@ def main():
     del GA, GB, GC
     torch.cuda.empty_cache()
 
+    batched = None
+    if not a.no_batched:
+        batched = {"f64": batched_workload(torch, tmv_amd, torch.float64),
+                   "f32": batched_workload(torch, tmv_amd, torch.float32)}
+
     cpu = None
     if not a.no_cpu_baseline:
         try:
@@ -228,6 +361,7 @@ def main():
                           "backward_error": be64},
         "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
         "dgemm_8192": gemm,
+        "batched_20000": batched,
         "roofline": {"bound": "mfma", "achieved": ccc["tflops"], "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": ccc["frac_peak"],
                      "traffic": DGEMM_8192_CCC_TRAFFIC if gn == 8192 else None,

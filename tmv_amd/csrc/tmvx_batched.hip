@@ -12,7 +12,9 @@
 //   cb + GD q (p, q < TR), so the shrinking trailing matrix stays spread
 //   over all threads.  The right-hand sides are one more column level
 //   (thread cb holds RHS column cb), so forward elimination of B happens
-//   with the factorisation.  Rows are never moved: pivoting is implicit,
+//   with the factorisation.  The column levels rotate through register
+//   slot 0 (one runtime loop body, not one per level), and the pivot row is
+//   read by a uniform register index.  Rows are never moved: pivoting is implicit,
 //   each wave keeps the position <-> row maps in registers and applies the
 //   interchange list's semantics (TMV's / LAPACK's piv[j]), so pivots, tie
 //   rule (first maximum in the current row order) and TMV's underflow rule
@@ -57,6 +59,7 @@ struct BLds {
     T urow[NMAX + GD];      // pivot row: columns, then right-hand sides
     T mult[NMAX];           // multipliers, indexed by row
     int pos[NMAX];          // final position of each row
+    int piv[NMAX];          // interchange list, written out at the end
 };
 
 struct GroupSync {
@@ -96,6 +99,13 @@ __device__ __forceinline__ int pick(const int (&v)[PR], int sel)
     return r;
 }
 
+// The TR rows a thread holds in one column level, as a vector so that the
+// pivot row (a uniform slot chosen at run time) is read with register-
+// indexed moves instead of a chain of selects.
+template <typename T, int TR> struct RowVec {
+    typedef T type __attribute__((ext_vector_type(TR)));
+};
+
 // Factor (and solve, nr > 0) one system with schedule S; tid in [0, S::NT).
 template <typename T, class S, class Sync>
 __device__ __forceinline__ void lu_solve_one(int n, int nr, T* __restrict__ A, i64* __restrict__ piv,
@@ -103,12 +113,17 @@ __device__ __forceinline__ void lu_solve_one(int n, int nr, T* __restrict__ A, i
                              BLds<T, S::NMAX, S::GD>& L, int tid, Sync sync)
 {
     constexpr int GD = S::GD, TR = S::TR, PR = S::PR, NW = S::NW;
+    typedef typename RowVec<T, TR>::type V;
     const int rb = tid / GD, cb = tid % GD;
     const int lane = tid & 63, wave = tid >> 6;
     const i64 ld = n;
 
-    T a[TR][TR];
-    T y[TR];
+    // a[s][p] = row rb + GD p of column cb + GD ((s + qj) mod TR): while
+    // column level qj is eliminated it sits in slot 0, the levels still to
+    // come in slots 1 .. TR-1-qj; the slots rotate once per level, so after
+    // TR levels every column is back in its own slot.
+    V a[TR];
+    V y;
     unsigned act = 0;        // bit p: row rb + GD p not yet pivotal
     static_for<0, TR>([&](auto pc) __attribute__((always_inline)) {
         constexpr int p = decltype(pc)::value;
@@ -116,7 +131,7 @@ __device__ __forceinline__ void lu_solve_one(int n, int nr, T* __restrict__ A, i
         static_for<0, TR>([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
             const int c = cb + GD * q;
-            a[p][q] = (r < n && c < n) ? A[r + c * ld] : T(0);
+            a[q][p] = (r < n && c < n) ? A[r + c * ld] : T(0);
         });
         y[p] = (r < n && cb < nr) ? B[r + cb * ld] : T(0);
         if (r < n) act |= 1u << p;
@@ -130,17 +145,16 @@ __device__ __forceinline__ void lu_solve_one(int n, int nr, T* __restrict__ A, i
     });
     int first_zero = 0;
 
-    static_for<0, TR>([&](auto qjc) __attribute__((always_inline)) {
-        constexpr int qj = decltype(qjc)::value;
+    for (int qj = 0; qj < TR; ++qj) {
         for (int jj = 0; jj < GD; ++jj) {
             const int j = qj * GD + jj;
             if (j >= n) break;
-            // 1. column j to LDS
+            // 1. column j (slot 0) to LDS
             if (cb == jj) {
                 static_for<0, TR>([&](auto pc) __attribute__((always_inline)) {
                     constexpr int p = decltype(pc)::value;
                     const int r = rb + GD * p;
-                    if (r < n) L.col[r] = a[p][qj];
+                    if (r < n) L.col[r] = a[0][p];
                 });
             }
             sync();
@@ -168,73 +182,71 @@ __device__ __forceinline__ void lu_solve_one(int n, int nr, T* __restrict__ A, i
                 if (k == rp) pos_r[s] = j; else if (k == rj) pos_r[s] = lp;
             });
             const T rcp = zero ? T(0) : T(1) / L.col[rp];
-            if (tid == 0) piv[j] = lp;
+            // (kept in LDS: a global store here would make every barrier
+            // wait for its completion)
+            if (tid == 0) L.piv[j] = lp;
             if (zero && first_zero == 0) first_zero = j + 1;
-            // 3. pivot row and multipliers to LDS.  Register rows are picked
-            //    with value selects (no stores under row-dependent branches),
-            //    which keeps the tile in registers.
+            // 3. pivot row (register slot pp, read by index) and multipliers
             const int pp = rp / GD;
             const bool own_piv = rb == rp % GD;
             if (own_piv) {
                 act &= ~(1u << pp);
-                static_for<0, TR>([&](auto qc) __attribute__((always_inline)) {
-                    constexpr int q = decltype(qc)::value;
-                    T v = T(0);
-                    static_for<0, TR>([&](auto pc) __attribute__((always_inline)) {
-                        constexpr int p = decltype(pc)::value;
-                        const T ap = a[p][q];
-                        v = (p == pp) ? ap : v;
-                    });
-                    const int c = cb + GD * q;
-                    if (c > j && c < n) L.urow[c] = v;
+                static_for<0, TR>([&](auto sc) __attribute__((always_inline)) {
+                    constexpr int s = decltype(sc)::value;
+                    if (s < TR - qj) {
+                        const int c = cb + GD * (s + qj);
+                        if (c > j && c < n) L.urow[c] = a[s][pp];
+                    }
                 });
-                if (cb < nr) {
-                    T v = T(0);
-                    static_for<0, TR>([&](auto pc) __attribute__((always_inline)) {
-                        constexpr int p = decltype(pc)::value;
-                        const T yp = y[p];
-                        v = (p == pp) ? yp : v;
-                    });
-                    L.urow[n + cb] = v;
-                }
+                if (cb < nr) L.urow[n + cb] = y[pp];
             }
             if (cb == jj) {
                 static_for<0, TR>([&](auto pc) __attribute__((always_inline)) {
                     constexpr int p = decltype(pc)::value;
                     const bool isact = (act >> p) & 1u;
                     const bool zp = zero && own_piv && p == pp;   // zeroed diagonal
-                    const T m = zero ? T(0) : a[p][qj] * rcp;
-                    a[p][qj] = (isact || zp) ? m : a[p][qj];
+                    const T m = zero ? T(0) : a[0][p] * rcp;
+                    a[0][p] = (isact || zp) ? m : a[0][p];
                     if (isact) L.mult[rb + GD * p] = m;
                 });
             }
             sync();
-            // 4. rank-1 update of the active rows (skipped for a zero column)
+            // 4. rank-1 update, unpredicated: pivoted rows get multiplier 0
+            //    and finished columns pivot-row value 0, so their FMAs add
+            //    exactly -0 (finite data; non-finite inputs give non-finite
+            //    factors, as in any LU).  Skipped for a zero column.
             if (!zero) {
-                T u[TR];
-                static_for<0, TR>([&](auto qc) __attribute__((always_inline)) {
-                    constexpr int q = decltype(qc)::value;
-                    const int c = cb + GD * q;
-                    u[q] = (c > j && c < n) ? L.urow[c] : T(0);
+                T l[TR];
+                static_for<0, TR>([&](auto pc) __attribute__((always_inline)) {
+                    constexpr int p = decltype(pc)::value;
+                    l[p] = ((act >> p) & 1u) ? L.mult[rb + GD * p] : T(0);
+                });
+                static_for<0, TR>([&](auto sc) __attribute__((always_inline)) {
+                    constexpr int s = decltype(sc)::value;
+                    if (s < TR - qj) {
+                        const int c = cb + GD * (s + qj);
+                        const T u = (c > j && c < n) ? L.urow[c] : T(0);
+                        static_for<0, TR>([&](auto pc) __attribute__((always_inline)) {
+                            constexpr int p = decltype(pc)::value;
+                            a[s][p] = fma(-l[p], u, a[s][p]);
+                        });
+                    }
                 });
                 const T uy = cb < nr ? L.urow[n + cb] : T(0);
                 static_for<0, TR>([&](auto pc) __attribute__((always_inline)) {
                     constexpr int p = decltype(pc)::value;
-                    if ((act >> p) & 1u) {
-                        const T l = L.mult[rb + GD * p];
-                        static_for<0, TR>([&](auto qc) __attribute__((always_inline)) {
-                            constexpr int q = decltype(qc)::value;
-                            if (GD * q + GD - 1 > j) {
-                                const int c = cb + GD * q;
-                                if (c > j && c < n) a[p][q] -= l * u[q];
-                            }
-                        });
-                        if (cb < nr) y[p] -= l * uy;
-                    }
+                    y[p] = fma(-l[p], uy, y[p]);
                 });
             }
         }
-    });
+        // next column level into slot 0
+        const V t0 = a[0];
+        static_for<0, TR - 1>([&](auto sc) __attribute__((always_inline)) {
+            constexpr int s = decltype(sc)::value;
+            a[s] = a[s + 1];
+        });
+        a[TR - 1] = t0;
+    }
 
     // Write the factors (and L^-1 P B) in pivoted row order.
     if (tid < 64) {
@@ -253,11 +265,12 @@ __device__ __forceinline__ void lu_solve_one(int n, int nr, T* __restrict__ A, i
             static_for<0, TR>([&](auto qc) __attribute__((always_inline)) {
                 constexpr int q = decltype(qc)::value;
                 const int c = cb + GD * q;
-                if (c < n) A[lp + c * ld] = a[p][q];
+                if (c < n) A[lp + c * ld] = a[q][p];
             });
             if (cb < nr) B[lp + cb * ld] = y[p];
         }
     });
+    for (int k = tid; k < n; k += S::NT) piv[k] = L.piv[k];
     if (tid == 0) *info = first_zero;
     sync();
     if (first_zero != 0) return;        // no solution for a singular system
