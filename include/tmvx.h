@@ -159,6 +159,24 @@ int tmvx_gesv_mixed_f64(int lowprec, int64_t n, int64_t nrhs, const double* A,
                         tmvx_ir_info* info, void* stream);
 
 /*
+ * Adjoint of the solve X = A^-1 B with the factors of tmvx_getrf_*: given
+ * the cotangent G = dL/dX (n x nrhs),
+ *   gradB = A^-T G                 (n x nrhs; may alias G)
+ *   gradA = -gradB X^T (+ gradA when accumulate != 0)      (n x n)
+ * No counterpart in TMV, which has no derivatives; the transposed solve
+ * is TMV's LU_RDivEq (x % A, src/TMV_LUDiv.cpp:369-410) and the rank-nrhs
+ * update its MultMM (src/TMV_MultMM.cpp:404-477).  Asynchronous.
+ */
+int tmvx_gesv_adjoint_f64(int64_t n, int64_t nrhs, const double* LU, int64_t lda,
+                          const int64_t* piv, const double* X, int64_t ldx, const double* G,
+                          int64_t ldg, double* gradB, int64_t ldgb, double* gradA, int64_t ldga,
+                          int accumulate, void* stream);
+int tmvx_gesv_adjoint_f32(int64_t n, int64_t nrhs, const float* LU, int64_t lda,
+                          const int64_t* piv, const float* X, int64_t ldx, const float* G,
+                          int64_t ldg, float* gradB, int64_t ldgb, float* gradA, int64_t ldga,
+                          int accumulate, void* stream);
+
+/*
  * Synthetic inputs: out[i] = lo + (hi-lo) * u(seed, offset + i), u in [0,1)
  * from the counter-based SplitMix64 stream documented in tmv_amd/rng.py
  * (bit-identical to it).  normal: Box-Muller over pairs of that stream.

@@ -92,6 +92,55 @@ int tmvx_gemm_batched_f32(int64_t count, const int32_t* m, const int32_t* n, con
                           float alpha, const float* const* A, const float* const* B, float beta,
                           float* const* C, void* stream);
 
+/*
+ * Factor a batch that may hold systems above TMVX_BATCH_NMAX: n_host (host)
+ * gives the orders; systems with n <= 128 run in one batched launch, larger
+ * ones through the blocked LU (tmvx_getrf_*), one after another.  Pivots and
+ * info as tmvx_gesv_batched_*.  Allocates and synchronises; not capturable.
+ * Per system TMV LU_Decompose (src/TMV_LUDecompose.cpp:514-559).
+ */
+int tmvx_getrf_batched_f64(int64_t count, const int32_t* n_host, double* const* A,
+                           int64_t* const* piv, int32_t* info, void* stream);
+int tmvx_getrf_batched_f32(int64_t count, const int32_t* n_host, float* const* A,
+                           int64_t* const* piv, int32_t* info, void* stream);
+
+/*
+ * Adjoint of X_i = A_i^-1 B_i for a batch (n_i <= 256), given stored
+ * factors (LU_i, piv_i), the solutions X_i and the cotangents G_i = dL/dX_i
+ * (all n_i x nrhs_i, leading dimension n_i):
+ *   gradB_i = A_i^-T G_i,   gradA_i = -gradB_i X_i^T (+ gradA_i if accumulate)
+ * One wave per system; the rank-nrhs update is fused into the kernel that
+ * solves for gradB_i, so gradA_i is written once.  n, nrhs are device
+ * arrays.  One launch, capturable.  No counterpart in TMV (no derivatives);
+ * per system the transposed solve is LU_RDivEq (src/TMV_LUDiv.cpp:369-410).
+ */
+int tmvx_gesv_adjoint_batched_f64(int64_t count, const int32_t* n, const int32_t* nrhs,
+                                  const double* const* LU, const int64_t* const* piv,
+                                  const double* const* X, const double* const* G,
+                                  double* const* gradB, double* const* gradA, int accumulate,
+                                  void* stream);
+int tmvx_gesv_adjoint_batched_f32(int64_t count, const int32_t* n, const int32_t* nrhs,
+                                  const float* const* LU, const int64_t* const* piv,
+                                  const float* const* X, const float* const* G,
+                                  float* const* gradB, float* const* gradA, int accumulate,
+                                  void* stream);
+/*
+ * The same adjoint without stored factors: A_i are the original matrices
+ * (not modified); their factors are recomputed into a workspace, for
+ * callers that drop the factors after the forward solve to save memory.
+ * n_host, nrhs_host are host arrays.  Allocates and synchronises.
+ */
+int tmvx_gesv_adjoint_batched_recompute_f64(int64_t count, const int32_t* n_host,
+                                            const int32_t* nrhs_host, const double* const* A,
+                                            const double* const* X, const double* const* G,
+                                            double* const* gradB, double* const* gradA,
+                                            int accumulate, void* stream);
+int tmvx_gesv_adjoint_batched_recompute_f32(int64_t count, const int32_t* n_host,
+                                            const int32_t* nrhs_host, const float* const* A,
+                                            const float* const* X, const float* const* G,
+                                            float* const* gradB, float* const* gradA,
+                                            int accumulate, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

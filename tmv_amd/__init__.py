@@ -65,6 +65,14 @@ _SIGS = {
     "tmvx_getrs_batched_f32": (_int, [_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "tmvx_gemm_batched_f64": (_int, [_i64, _vp, _vp, _vp, _d, _vp, _vp, _d, _vp, _vp]),
     "tmvx_gemm_batched_f32": (_int, [_i64, _vp, _vp, _vp, _f, _vp, _vp, _f, _vp, _vp]),
+    "tmvx_getrf_batched_f64": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
+    "tmvx_getrf_batched_f32": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
+    "tmvx_gesv_adjoint_f64": (_int, [_i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _int, _vp]),
+    "tmvx_gesv_adjoint_f32": (_int, [_i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _int, _vp]),
+    "tmvx_gesv_adjoint_batched_f64": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp]),
+    "tmvx_gesv_adjoint_batched_f32": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp]),
+    "tmvx_gesv_adjoint_batched_recompute_f64": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp]),
+    "tmvx_gesv_adjoint_batched_recompute_f32": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp]),
     # tuning hooks (include/tmvx_tune.h)
     "tmvx_gemm_config_count": (_int, [_int]),
     "tmvx_gemm_config_name": (ctypes.c_char_p, [_int, _int]),
@@ -318,7 +326,8 @@ class Batch:
     leading dimension n_i, plus the device pointer arrays the C ABI takes.
     ``a(i)`` / ``b(i)`` / ``piv(i)`` are views of system i."""
 
-    def __init__(self, ns, nrhs=None, dtype=None, device="cuda"):
+    def __init__(self, ns, nrhs=None, dtype=None, device="cuda", A=None, B=None):
+        """A, B: existing flat device buffers to wrap (else allocated)."""
         torch = _torch()
         import numpy as np
         dtype = torch.float64 if dtype is None else dtype
@@ -335,8 +344,12 @@ class Batch:
         self.offB = np.concatenate([[0], np.cumsum(nb)[:-1]]).astype(np.int64) if cnt else nb
         self.offP = (np.concatenate([[0], np.cumsum(self.ns)[:-1]]).astype(np.int64)
                      if cnt else self.ns.astype(np.int64))
-        self.A = torch.zeros(max(int(na.sum()), 1), dtype=dtype, device=device)
-        self.B = torch.zeros(max(int(nb.sum()), 1), dtype=dtype, device=device)
+        self.A = (torch.zeros(max(int(na.sum()), 1), dtype=dtype, device=device)
+                  if A is None else A)
+        self.B = (torch.zeros(max(int(nb.sum()), 1), dtype=dtype, device=device)
+                  if B is None else B)
+        if self.A.numel() < int(na.sum()) or self.B.numel() < int(nb.sum()):
+            raise ValueError("buffer smaller than the batch layout")
         self.P = torch.zeros(max(int(self.ns.sum()), 1), dtype=torch.int64, device=device)
         self.info = torch.zeros(max(cnt, 1), dtype=torch.int32, device=device)
         es = self.A.element_size()
@@ -429,3 +442,61 @@ def gemm_batched(m, n, k, pA, pB, pC, alpha=1.0, beta=0.0, dtype=None, stream=No
     fn = getattr(lib(), "tmvx_gemm_batched_" + _suffix(dtype))
     _check(fn(m.numel(), m.data_ptr(), n.data_ptr(), k.data_ptr(), alpha, pA.data_ptr(),
               pB.data_ptr(), beta, pC.data_ptr(), _stream(stream)), "gemm_batched")
+
+
+def getrf_batched(batch: Batch, stream=None):
+    """Factor every system of ``batch`` in place (any n: n <= 128 batched,
+    larger ones through the blocked LU).  Synchronises; returns info."""
+    import numpy as np
+    fn = getattr(lib(), "tmvx_getrf_batched_" + _suffix(batch.dtype))
+    nh = np.ascontiguousarray(batch.ns, dtype=np.int32)
+    _check(fn(batch.count, nh.ctypes.data, batch.pA.data_ptr(), batch.pP.data_ptr(),
+              batch.info.data_ptr(), _stream(stream)), "getrf_batched")
+    return batch.info
+
+
+def gesv_adjoint(LU, piv, X, G, gradA=None, accumulate=False, stream=None):
+    """gradB = A^-T G, gradA (+)= -gradB X^T for one system with the factors
+    of getrf.  X, G n x nrhs column-major (1-D = one column).  Returns
+    (gradB, gradA)."""
+    torch = _torch()
+    n = LU.shape[0]
+    lda = _colmajor_ld(LU, "LU")
+    Xm = X.view(n, 1) if X.dim() == 1 else X
+    Gm = G.view(n, 1) if G.dim() == 1 else G
+    nrhs = Xm.shape[1]
+    ldx = _colmajor_ld(Xm, "X") if nrhs > 1 else max(n, 1)
+    ldg = _colmajor_ld(Gm, "G") if nrhs > 1 else max(n, 1)
+    gB = empty_colmajor(n, nrhs, LU.dtype, LU.device)
+    if gradA is None:
+        gradA = empty_colmajor(n, n, LU.dtype, LU.device)
+        accumulate = False
+    ldga = _colmajor_ld(gradA, "gradA")
+    fn = getattr(lib(), "tmvx_gesv_adjoint_" + _suffix(LU.dtype))
+    _check(fn(n, nrhs, LU.data_ptr(), lda, piv.data_ptr(), Xm.data_ptr(), ldx, Gm.data_ptr(),
+              ldg, gB.data_ptr(), max(n, 1), gradA.data_ptr(), ldga, int(bool(accumulate)),
+              _stream(stream)), "gesv_adjoint")
+    return (gB if X.dim() > 1 else gB.view(n)), gradA
+
+
+def gesv_adjoint_batched(fact: Batch, X: Batch, G: Batch, gradB: Batch, gradA: Batch,
+                         accumulate=False, recompute=False, stream=None):
+    """Batched adjoint.  ``fact`` holds the factors in fact.A / fact.P
+    (recompute=False) or the original matrices in fact.A (recompute=True);
+    X, G, gradB are batches whose B buffers hold the n_i x nrhs_i blocks;
+    gradA's A buffer receives the n_i x n_i gradients."""
+    import numpy as np
+    suf = _suffix(fact.dtype)
+    if recompute:
+        fn = getattr(lib(), "tmvx_gesv_adjoint_batched_recompute_" + suf)
+        nh = np.ascontiguousarray(fact.ns, dtype=np.int32)
+        rh = np.ascontiguousarray(X.nrhs, dtype=np.int32)
+        _check(fn(fact.count, nh.ctypes.data, rh.ctypes.data, fact.pA.data_ptr(),
+                  X.pB.data_ptr(), G.pB.data_ptr(), gradB.pB.data_ptr(), gradA.pA.data_ptr(),
+                  int(bool(accumulate)), _stream(stream)), "gesv_adjoint_batched_recompute")
+    else:
+        fn = getattr(lib(), "tmvx_gesv_adjoint_batched_" + suf)
+        _check(fn(fact.count, fact.d_n.data_ptr(), X.d_nrhs.data_ptr(), fact.pA.data_ptr(),
+                  fact.pP.data_ptr(), X.pB.data_ptr(), G.pB.data_ptr(), gradB.pB.data_ptr(),
+                  gradA.pA.data_ptr(), int(bool(accumulate)), _stream(stream)),
+               "gesv_adjoint_batched")

@@ -34,6 +34,7 @@
 // captured in a hipGraph and replayed.
 #include "tmvx_lu.h"
 #include "tmvx_pivot.h"
+#include "tmvx_wave_solve.h"
 #include "../../include/tmvx_batched.h"
 
 #include <algorithm>
@@ -363,37 +364,6 @@ batched_lu_solve_kernel(BatchArgs<T> args)
 // ---------------------------------------------------------------------------
 // Batched solve with stored factors: one wave per (system, right-hand side).
 // ---------------------------------------------------------------------------
-// Wave sum by DPP, result in every lane (same combining pattern as the
-// argmax: quad swaps, mirrors, row broadcasts; masked-off rows add zero).
-template <int CTRL, int RM>
-__device__ __forceinline__ double dpp_add(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, RM, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, RM, 0xf, false);
-    return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-template <int CTRL, int RM>
-__device__ __forceinline__ float dpp_add(float v)
-{
-    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, RM, 0xf,
-                                                          false));
-}
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v)
-{
-    v = dpp_add<0xB1, 0xf>(v);
-    v = dpp_add<0x4E, 0xf>(v);
-    v = dpp_add<0x141, 0xf>(v);
-    v = dpp_add<0x140, 0xf>(v);
-    v = dpp_add<0x142, 0xa>(v);
-    v = dpp_add<0x143, 0xc>(v);
-    return readlane(v, 63);
-}
-
-constexpr int RS_NMAX = 256;      // largest order of the stored-factor solve
-constexpr int RS_PR = RS_NMAX / 64;
-
 template <typename T>
 __global__ void __launch_bounds__(256)
 batched_getrs_kernel(int trans, int count, const int* __restrict__ nv,
@@ -404,106 +374,14 @@ batched_getrs_kernel(int trans, int count, const int* __restrict__ nv,
     const int gw = (int)((blockIdx.x * (i64)blockDim.x + threadIdx.x) >> 6);
     const int nwaves = (int)((gridDim.x * (i64)blockDim.x) >> 6);
     for (int sys = gw; sys < count; sys += nwaves) {
-        const int nr = nrhsv[sys];
-        if (nv[sys] < 0 || nv[sys] > RS_NMAX) continue;
+        const int n = nv[sys], nr = nrhsv[sys];
+        if (n < 0 || n > WS_NMAX) continue;
         for (int c = 0; c < nr; ++c) {
-            const int n = nv[sys];
-            const T* LU = LUv[sys];
-            const i64* piv = pivv[sys];
             T* b = Bv[sys] + (i64)c * n;
-            const i64 ld = n;
-            T x[RS_PR];
-            static_for<0, RS_PR>([&](auto sc) __attribute__((always_inline)) {
-                constexpr int s = decltype(sc)::value;
-                const int i = lane + 64 * s;
-                x[s] = i < n ? b[i] : T(0);
-            });
-            // Interchange positions j and piv[j] in the lane-distributed x.
-            auto swap_pos = [&](int j, int p) {
-                if (p == j || p < 0 || p >= n) return;
-                T xj = x[0], xp = x[0];
-                static_for<1, RS_PR>([&](auto sc) __attribute__((always_inline)) {
-                    constexpr int s = decltype(sc)::value;
-                    if ((j >> 6) == s) xj = x[s];
-                    if ((p >> 6) == s) xp = x[s];
-                });
-                xj = readlane(xj, j & 63);
-                xp = readlane(xp, p & 63);
-                static_for<0, RS_PR>([&](auto sc) __attribute__((always_inline)) {
-                    constexpr int s = decltype(sc)::value;
-                    const int i = lane + 64 * s;
-                    if (i == j) x[s] = xp; else if (i == p) x[s] = xj;
-                });
-            };
-            auto get = [&](int k) -> T {
-                T v = x[0];
-                static_for<1, RS_PR>([&](auto sc) __attribute__((always_inline)) {
-                    constexpr int s = decltype(sc)::value;
-                    if ((k >> 6) == s) v = x[s];
-                });
-                return readlane(v, k & 63);
-            };
-            auto set = [&](int k, T v) {
-                static_for<0, RS_PR>([&](auto sc) __attribute__((always_inline)) {
-                    constexpr int s = decltype(sc)::value;
-                    if (lane + 64 * s == k) x[s] = v;
-                });
-            };
-            if (!trans) {
-                for (int j = 0; j < n; ++j) swap_pos(j, (int)piv[j]);
-                // L (unit lower), column sweep
-                for (int k = 0; k < n; ++k) {
-                    const T xk = get(k);
-                    if (xk != T(0)) {
-                        static_for<0, RS_PR>([&](auto sc) __attribute__((always_inline)) {
-                            constexpr int s = decltype(sc)::value;
-                            const int i = lane + 64 * s;
-                            if (i > k && i < n) x[s] -= LU[i + (i64)k * ld] * xk;
-                        });
-                    }
-                }
-                // U, column sweep from the bottom
-                for (int k = n - 1; k >= 0; --k) {
-                    const T yk = get(k);
-                    const T xk = yk == T(0) ? T(0) : yk / LU[k + (i64)k * ld];
-                    set(k, xk);
-                    if (xk != T(0)) {
-                        static_for<0, RS_PR>([&](auto sc) __attribute__((always_inline)) {
-                            constexpr int s = decltype(sc)::value;
-                            const int i = lane + 64 * s;
-                            if (i < k) x[s] -= LU[i + (i64)k * ld] * xk;
-                        });
-                    }
-                }
-            } else {
-                // U^T (lower): z_i = (g_i - sum_{k<i} U(k,i) z_k) / U(i,i)
-                for (int i = 0; i < n; ++i) {
-                    T part = T(0);
-                    static_for<0, RS_PR>([&](auto sc) __attribute__((always_inline)) {
-                        constexpr int s = decltype(sc)::value;
-                        const int k = lane + 64 * s;
-                        if (k < i) part += LU[k + (i64)i * ld] * x[s];
-                    });
-                    const T gi = get(i) - wave_sum(part);
-                    set(i, gi == T(0) ? T(0) : gi / LU[i + (i64)i * ld]);
-                }
-                // L^T (unit upper): w_i = z_i - sum_{k>i} L(k,i) w_k
-                for (int i = n - 1; i >= 0; --i) {
-                    T part = T(0);
-                    static_for<0, RS_PR>([&](auto sc) __attribute__((always_inline)) {
-                        constexpr int s = decltype(sc)::value;
-                        const int k = lane + 64 * s;
-                        if (k > i && k < n) part += LU[k + (i64)i * ld] * x[s];
-                    });
-                    set(i, get(i) - wave_sum(part));
-                }
-                for (int j = n - 1; j >= 0; --j) swap_pos(j, (int)piv[j]);
-            }
-            static_for<0, RS_PR>([&](auto sc) __attribute__((always_inline)) {
-                constexpr int s = decltype(sc)::value;
-                const int i = lane + 64 * s;
-                if (i < n) b[i] = x[s];
-            });
+            T x[WS_PR];
+            wave_load(x, b, n, lane);
+            wave_getrs(trans, n, LUv[sys], (i64)n, pivv[sys], x, lane);
+            wave_store(x, b, n, lane);
         }
     }
 }
@@ -950,7 +828,7 @@ static int getrs_batched_launch(int trans, int64_t count, const int32_t* n, cons
     if (count < 0 || count > INT_MAX) return TMVX_ERR_ARG;
     if (count == 0) return TMVX_OK;
     if (!n || !nrhs || !LU || !piv || !B) return TMVX_ERR_ARG;
-    const unsigned grid = 2048;      // 8192 waves; each takes every 8192-th (system, column)
+    const unsigned grid = 2048;      // 8192 waves; wave w takes systems w, w + 8192, ...
     if (f64)
         hipLaunchKernelGGL(batched_getrs_kernel<double>, dim3(grid), dim3(256), 0, s, trans,
                            (int)count, n, nrhs, (const double* const*)LU, (const i64* const*)piv,
