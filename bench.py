@@ -21,6 +21,9 @@ Also measured in the same run (outside the K timed steps):
     diagonal), nrhs ~ U{1..4}, FP64 and FP32, factor + solve in one
     hipGraph-replayed persistent launch (autotuned plan) vs per-matrix
     launches, plus the batched GEMM with K = n;
+  - the differentiation workload: 512 systems n = 256, forward solve and
+    the adjoint (grad_b = A^-T g, grad_A = -grad_b x^T) with stored and
+    with recomputed factors;
   - the CPU baseline: the real TMV 0.76 library (oracle/_ref, built from
     the reference sources) on a bounded sample, x = b/A at n = 4096 with
     OpenMP on the host's cores (kind "reference"); the oracle restatement
@@ -221,6 +224,71 @@ def batched_workload(torch, tmv_amd, dtype, count=20000, reps=5, per_matrix_samp
     return out
 
 
+def adjoint_workload(torch, tmv_amd, dtype, count=512, n=256, reps=3):
+    """512 systems, n = 256: A = Q diag(s) Q^T + n I (Q from the QR of a
+    seeded Gaussian, s ~ U[0.1, 10]), b, w ~ N(0, 1), loss = sum(w .* x).
+    Forward: batched factorisation + solve; backward: the adjoint kernel
+    with stored factors, and with recomputed factors."""
+    import numpy as np
+    ns = np.full(count, n, np.int32)
+    nr = np.ones(count, np.int32)
+    b = tmv_amd.Batch(ns, nr, dtype=dtype)
+    g = torch.empty(n * n, dtype=torch.float64, device="cuda")
+    tmv_amd.fill_normal(g, SEED + 50)
+    Q, _ = torch.linalg.qr(g.view(n, n))
+    s = torch.empty(count * n, dtype=torch.float64, device="cuda")
+    tmv_amd.fill_uniform(s, SEED + 51, 0.1, 10.0)
+    Qs = Q.unsqueeze(0) * s.view(count, 1, n)                  # Q diag(s_b)
+    A3 = torch.matmul(Qs, Q.t().contiguous()) + n * torch.eye(
+        n, dtype=torch.float64, device="cuda")
+    del Qs
+    A0 = A3.transpose(1, 2).contiguous().view(-1).to(dtype)     # column-major blocks
+    del A3, g
+    B0 = torch.empty(count * n, dtype=torch.float64, device="cuda")
+    tmv_amd.fill_normal(B0, SEED + 52)
+    B0 = B0.to(dtype)
+    W = torch.empty(count * n, dtype=torch.float64, device="cuda")
+    tmv_amd.fill_normal(W, SEED + 53)
+    W = W.to(dtype)
+    fwd = []
+    for _ in range(reps):
+        b.A.copy_(A0); b.B.copy_(B0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tmv_amd.getrf_batched(b)
+        tmv_amd.getrs_batched(b)
+        torch.cuda.synchronize()
+        fwd.append(time.perf_counter() - t0)
+    Xb = tmv_amd.Batch(ns, nr, dtype=dtype, B=b.B.clone())
+    Gb = tmv_amd.Batch(ns, nr, dtype=dtype, B=W)
+    gB = tmv_amd.Batch(ns, nr, dtype=dtype)
+    gA = tmv_amd.Batch(ns, nr, dtype=dtype)
+    bwd = []
+    for _ in range(reps + 1):
+        e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        tmv_amd.gesv_adjoint_batched(b, Xb, Gb, gB, gA)
+        e1.record()
+        torch.cuda.synchronize()
+        bwd.append(e0.elapsed_time(e1))
+    orig = tmv_amd.Batch(ns, nr, dtype=dtype, A=A0)
+    rec = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tmv_amd.gesv_adjoint_batched(orig, Xb, Gb, gB, gA, recompute=True)
+        torch.cuda.synchronize()
+        rec.append(time.perf_counter() - t0)
+    ms = min(bwd[1:])
+    esz = b.A.element_size()
+    bytes_ = float(count * (2 * n * n * esz + 3 * n * esz + n * 8))   # LU + gradA, vectors, piv
+    return {"systems": count, "n": n, "dtype": str(dtype).replace("torch.", ""),
+            "forward_ms": round(min(fwd) * 1e3, 2), "adjoint_ms": round(ms, 3),
+            "adjoint_gbs": round(bytes_ / ms / 1e6, 1),
+            "adjoint_hbm_frac": round(bytes_ / ms / 1e6 / 8000.0, 4),
+            "adjoint_recompute_ms": round(min(rec) * 1e3, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,10 +313,13 @@ def main():
     import tmv_amd
     tmv_amd.lib()                      # the HIP library; no fallback
     n, nrhs = a.n, a.nrhs
-    if a.only_batched:                 # the batched workload alone (profiling)
+    if a.only_batched:                 # the batched workloads alone (profiling)
         print(json.dumps({"batched_20000": {
             "f64": batched_workload(torch, tmv_amd, torch.float64),
-            "f32": batched_workload(torch, tmv_amd, torch.float32)}}), flush=True)
+            "f32": batched_workload(torch, tmv_amd, torch.float32)},
+            "adjoint_512x256": {
+            "f64": adjoint_workload(torch, tmv_amd, torch.float64),
+            "f32": adjoint_workload(torch, tmv_amd, torch.float32)}}), flush=True)
         return
 
     # ---- inputs (untimed) -------------------------------------------------
@@ -322,10 +393,12 @@ def main():
     del GA, GB, GC
     torch.cuda.empty_cache()
 
-    batched = None
+    batched = adjoint = None
     if not a.no_batched:
         batched = {"f64": batched_workload(torch, tmv_amd, torch.float64),
                    "f32": batched_workload(torch, tmv_amd, torch.float32)}
+        adjoint = {"f64": adjoint_workload(torch, tmv_amd, torch.float64),
+                   "f32": adjoint_workload(torch, tmv_amd, torch.float32)}
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -362,6 +435,7 @@ def main():
         "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
         "dgemm_8192": gemm,
         "batched_20000": batched,
+        "adjoint_512x256": adjoint,
         "roofline": {"bound": "mfma", "achieved": ccc["tflops"], "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": ccc["frac_peak"],
                      "traffic": DGEMM_8192_CCC_TRAFFIC if gn == 8192 else None,

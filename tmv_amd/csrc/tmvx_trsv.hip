@@ -12,7 +12,7 @@
 // ready flags appear, then solves its diagonal block with one lane per row
 // (row of the diagonal block in registers, x_k broadcast by readlane),
 // publishes x_b with device-coherent stores and raises its flag.  Up to
-// SV_RHS right-hand sides per launch.  Every wait is bounded.
+// SV_RHS right-hand sides per launch, one wave each.  Every wait is bounded.
 #include "tmvx_lu.h"
 
 namespace tmvx {
@@ -40,6 +40,15 @@ struct SvWork {
 // written to Xout through `out_perm` (scatter), nr <= SV_RHS columns.
 // `lower`: op(A) lower triangular (forward order), else upper (backward).
 // op(A)(i, j) = trans ? A[j + i*lda] : A[i + j*lda].
+//
+// Wave w of the block's workgroup owns right-hand side w; lane i owns row
+// b0 + i.  A finished block's x arrives as one coherent load per lane and
+// is broadcast with readlane, so contributions need no barrier.  Ready
+// flags are checked in bulk: one poll reads the flags of the next 64
+// blocks in solve order and the longest finished prefix is consumed
+// without further polls (contributions stay in solve order, so results
+// are deterministic).  The diagonal block and the reciprocals of its
+// diagonal are loaded first, off the dependency chain.
 template <typename T>
 __global__ void __launch_bounds__(256)
 trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
@@ -52,106 +61,115 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;   // wave = RHS column
     const i64 nblk = (n + NB - 1) / NB;
     __shared__ unsigned s_tk;
-    __shared__ T s_x[SV_RHS][NB];      // x of the block being accumulated
-    __shared__ int s_fail;
-    if (t == 0) {
-        s_tk = atomicAdd(wk.ticket, 1u);
-        s_fail = 0;
-    }
+    if (t == 0) s_tk = atomicAdd(wk.ticket, 1u);
     __syncthreads();
     const i64 q = s_tk;                       // position in solve order
     if (q >= nblk) return;
     const i64 blk = lower ? q : nblk - 1 - q;
     const i64 b0 = blk * NB;
     const int bs = (int)((n - b0) < NB ? (n - b0) : NB);
-    const bool has = wave < nr;
+    const bool row_ok = lane < bs;
     auto opA = [&](i64 i, i64 j) -> T { return trans ? A[j + i * lda] : A[i + j * lda]; };
 
-    // Right-hand side rows of this block.
-    T acc = T(0);
-    if (has && lane < bs) {
-        i64 src = b0 + lane;
-        if (in_perm) { src = in_perm[b0 + lane]; if (src < 0 || src >= n) src = b0 + lane; }
-        acc = Bin[src + wave * ldb];
-    }
-    // Contributions of the finished blocks, in solve order.  The A block
-    // (64 values per lane) is loaded before waiting for the block's flag,
-    // so its latency overlaps the wait.
-    for (i64 qc = 0; qc < q; ++qc) {
-        const i64 cb = lower ? qc : nblk - 1 - qc;
-        const i64 c0 = cb * NB;
-        const int cs = (int)((n - c0) < NB ? (n - c0) : NB);
-        T ablk[NB];
-        static_for<0, NB>([&](auto kc) {
+    if (wave < nr) {
+        // diagonal block row and reciprocal of the diagonal, loaded first
+        T drow[NB];
+        static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
             constexpr int k = decltype(kc)::value;
-            ablk[k] = (has && lane < bs && k < cs) ? opA(b0 + lane, c0 + k) : T(0);
+            drow[k] = (row_ok && k < bs) ? opA(b0 + lane, b0 + k) : T(0);
         });
-        if (t == 0) {
-            unsigned spins = 0;
-            while (__hip_atomic_load(wk.flags + cb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                   != wk.epoch) {
-                if (++spins > SV_SPIN_LIMIT ||
-                    __hip_atomic_load(wk.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    __hip_atomic_store(wk.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    atomicOr(wk.singular, 2);      // bit 1: a bounded wait expired
-                    s_fail = 1;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
+        T rdiag = T(1);
+        if (!unit && row_ok) {
+            const T d = opA(b0 + lane, b0 + lane);
+            rdiag = d == T(0) ? T(0) : T(1) / d;
         }
-        __syncthreads();
-        if (s_fail) return;
-        if (has) s_x[wave][lane] = lane < cs ? sv_cload(xpub + c0 + lane + (i64)wave * n) : T(0);
-        __syncthreads();
-        if (has && lane < bs) {
+        T acc = T(0);
+        if (row_ok) {
+            i64 src = b0 + lane;
+            if (in_perm) { src = in_perm[b0 + lane]; if (src < 0 || src >= n) src = b0 + lane; }
+            acc = Bin[src + wave * ldb];
+        }
+        bool failed = false;
+        i64 ready = 0;                        // blocks [0, ready) in solve order are done
+        for (i64 qc = 0; qc < q && !failed; ++qc) {
+            const i64 cb = lower ? qc : nblk - 1 - qc;
+            const i64 c0 = cb * NB;
+            const int cs = (int)((n - c0) < NB ? (n - c0) : NB);
+            T ablk[NB];
+            static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
+                constexpr int k = decltype(kc)::value;
+                ablk[k] = (row_ok && k < cs) ? opA(b0 + lane, c0 + k) : T(0);
+            });
+            if (qc >= ready) {
+                unsigned spins = 0;
+                while (true) {
+                    const i64 qq = qc + lane;
+                    bool done = true;
+                    if (qq < q) {
+                        const i64 fb = lower ? qq : nblk - 1 - qq;
+                        done = __hip_atomic_load(wk.flags + fb, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT) == wk.epoch;
+                    }
+                    const unsigned long long bal = __ballot(done);
+                    const int run = bal == ~0ull ? 64 : __builtin_ctzll(~bal);
+                    if (run > 0) { ready = qc + run; break; }
+                    if (++spins > SV_SPIN_LIMIT ||
+                        __hip_atomic_load(wk.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        if (lane == 0) {
+                            __hip_atomic_store(wk.err, 1, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                            atomicOr(wk.singular, 2);      // bit 1: a bounded wait expired
+                        }
+                        failed = true;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                if (failed) break;
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            const T xc = lane < cs ? sv_cload(xpub + c0 + lane + (i64)wave * n) : T(0);
             T s0 = T(0), s1 = T(0);
-            static_for<0, NB / 2>([&](auto kc) {
+            static_for<0, NB / 2>([&](auto kc) __attribute__((always_inline)) {
                 constexpr int k = 2 * decltype(kc)::value;
-                s0 += ablk[k] * s_x[wave][k];
-                s1 += ablk[k + 1] * s_x[wave][k + 1];
+                s0 += ablk[k] * readlane(xc, k);
+                s1 += ablk[k + 1] * readlane(xc, k + 1);
             });
             acc -= s0 + s1;
         }
-        __syncthreads();
-    }
-    // Diagonal block: lane i keeps row i of op(A)[b, b].
-    T drow[NB];
-    static_for<0, NB>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        drow[k] = (lane < bs && k < bs) ? opA(b0 + lane, b0 + k) : T(0);
-    });
-    T x = acc;
-    bool sing = false;
-    if (lower) {
-        static_for<0, NB>([&](auto kc) {
-            constexpr int k = decltype(kc)::value;
-            if (k < bs) {
-                if (!unit && lane == k && x != T(0)) {
-                    if (drow[k] == T(0)) sing = true; else x = x / drow[k];
+        // Diagonal block: lane i keeps row i of op(A)[b, b].
+        T x = acc;
+        bool sing = false;
+        if (lower) {
+            static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
+                constexpr int k = decltype(kc)::value;
+                if (k < bs) {
+                    if (!unit && lane == k && x != T(0)) {
+                        if (rdiag == T(0)) sing = true; else x = x * rdiag;
+                    }
+                    const T xk = readlane(x, k);
+                    if (lane > k) x -= drow[k] * xk;
                 }
-                const T xk = readlane(x, k);
-                if (lane > k) x -= drow[k] * xk;
-            }
-        });
-    } else {
-        static_for<0, NB>([&](auto kc) {
-            constexpr int k = NB - 1 - decltype(kc)::value;
-            if (k < bs) {
-                if (!unit && lane == k && x != T(0)) {
-                    if (drow[k] == T(0)) sing = true; else x = x / drow[k];
+            });
+        } else {
+            static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
+                constexpr int k = NB - 1 - decltype(kc)::value;
+                if (k < bs) {
+                    if (!unit && lane == k && x != T(0)) {
+                        if (rdiag == T(0)) sing = true; else x = x * rdiag;
+                    }
+                    const T xk = readlane(x, k);
+                    if (lane < k) x -= drow[k] * xk;
                 }
-                const T xk = readlane(x, k);
-                if (lane < k) x -= drow[k] * xk;
-            }
-        });
-    }
-    if (has && sing) atomicOr(wk.singular, 1);
-    if (has && lane < bs) {
-        sv_cstore(xpub + b0 + lane + (i64)wave * n, x);
-        i64 dst = b0 + lane;
-        if (out_perm) { dst = out_perm[b0 + lane]; if (dst < 0 || dst >= n) dst = b0 + lane; }
-        Xout[dst + wave * ldx] = x;
+            });
+        }
+        if (sing) atomicOr(wk.singular, 1);
+        if (row_ok && !failed) {
+            sv_cstore(xpub + b0 + lane + (i64)wave * n, x);
+            i64 dst = b0 + lane;
+            if (out_perm) { dst = out_perm[b0 + lane]; if (dst < 0 || dst >= n) dst = b0 + lane; }
+            Xout[dst + wave * ldx] = x;
+        }
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
