@@ -28,7 +28,9 @@
 //      the panel as a one-wave-per-column gather; the composed permutation
 //      vector is updated the same way.
 //   3. trsm_u12_kernel: U12 = L11^-1 A12 (lane i owns row i of L11).
-//   4. gemm_launch: A22 -= A21 * U12 on MFMA.
+//   4. gemm_launch: A22 -= A21 * U12 on MFMA, with look-ahead: the next
+//      outer panel's columns are updated first on the caller's stream,
+//      the rest on a second stream while the next panel is factored.
 #include "tmvx_lu.h"
 #include "tmvx_pivot.h"
 #include <cfloat>
@@ -480,30 +482,55 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
         rc = hip_status(hipGetLastError());
     }
     // Interchanges of panel [j0, j0+jb) on columns [a0,a1) u [b0,b1).
-    auto swap_cols = [&](i64 j0, int jb, i64 a0, i64 a1, i64 b0, i64 b1, i64* pm) -> int {
+    auto swap_cols = [&](i64 j0, int jb, i64 a0, i64 a1, i64 b0, i64 b1, i64* pm,
+                         hipStream_t st) -> int {
         const i64 cols = (a1 - a0) + (b1 - b0);
         if (cols <= 0 && !pm) return TMVX_OK;
         const int cpw = 4;
         const i64 waves = (cols + cpw - 1) / cpw;
         const unsigned grid = (unsigned)(waves > 0 ? (waves + 3) / 4 : 1);
-        hipLaunchKernelGGL((laswp_panel_kernel<T, NB>), dim3(grid), dim3(256), 0, s, A, lda, n,
+        hipLaunchKernelGGL((laswp_panel_kernel<T, NB>), dim3(grid), dim3(256), 0, st, A, lda, n,
                            a0, a1, b0, b1, j0, jb, piv, pm, cpw);
         return hip_status(hipGetLastError());
     };
     // Rows [j0, j0+jb) of columns [c0, c0+nc) := L(j0..)^-1 times themselves.
-    auto solve_rows = [&](i64 j0, int jb, i64 c0, i64 nc) -> int {
+    auto solve_rows = [&](i64 j0, int jb, i64 c0, i64 nc, hipStream_t st) -> int {
         if (nc <= 0) return TMVX_OK;
         const int cpw = 16;
         const i64 waves = (nc + cpw - 1) / cpw;
         hipLaunchKernelGGL((trsm_u12_kernel<T, NB>), dim3((unsigned)((waves + 3) / 4)),
-                           dim3(256), 0, s, A, lda, j0, jb, c0, nc, cpw);
+                           dim3(256), 0, st, A, lda, j0, jb, c0, nc, cpw);
         return hip_status(hipGetLastError());
+    };
+    // U12 of the outer block row [J0, JE) for columns [c0, c1): NB rows at
+    // a time, each followed by the update of the block row's rows below it.
+    auto block_row_u12 = [&](i64 J0, i64 JE, i64 c0, i64 c1, hipStream_t st) -> int {
+        for (i64 j0 = J0; j0 < JE; j0 += NB) {
+            const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
+            TMVX_TRY(solve_rows(j0, jb, c0, c1 - c0, st));
+            const i64 below = JE - j0 - jb;
+            if (below > 0)
+                TMVX_TRY(gemm_launch<T>(0, 0, below, c1 - c0, jb, T(-1), A + (j0 + jb) + j0 * lda,
+                                        lda, A + j0 + c0 * lda, lda, T(1),
+                                        A + (j0 + jb) + c0 * lda, lda, st));
+        }
+        return TMVX_OK;
     };
     // Two-level blocking: outer panels of NBO columns are factorised as
     // NB-column panels updated only inside the outer panel; the trailing
     // matrix then takes one update with K = NBO (4x the arithmetic per
     // byte of C moved compared with K = NB).
+    //
+    // Look-ahead: after outer panel k, the caller's stream updates only the
+    // next outer panel's columns and goes on to factor it, while a second
+    // stream updates the rest of the trailing matrix.  The panel chain (a
+    // few workgroups, latency-bound) then overlaps the big GEMM.  The next
+    // panel's interchanges outside it wait for that GEMM (it reads the L
+    // and U blocks the interchanges would move).
     constexpr i64 NBO = 4 * NB;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_main = nullptr, ev_side = nullptr;
+    bool side_pending = false;
     for (i64 J0 = 0; J0 < n && rc == TMVX_OK; J0 += NBO) {
         const i64 JE = (n - J0) < NBO ? n : J0 + NBO;      // outer panel [J0, JE)
         for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
@@ -513,37 +540,69 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
                                A, lda, n, j0, jb, piv, wk);
             if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
             // inside the outer panel only; the rest is deferred
-            if ((rc = swap_cols(j0, jb, J0, j0, j0 + jb, JE, perm)) != TMVX_OK) break;
+            if ((rc = swap_cols(j0, jb, J0, j0, j0 + jb, JE, perm, s)) != TMVX_OK) break;
             const i64 nr = JE - j0 - jb;
             if (nr > 0) {
-                if ((rc = solve_rows(j0, jb, j0 + jb, nr)) != TMVX_OK) break;
+                if ((rc = solve_rows(j0, jb, j0 + jb, nr, s)) != TMVX_OK) break;
                 rc = gemm_launch<T>(0, 0, n - j0 - jb, nr, jb, T(-1), A + (j0 + jb) + j0 * lda,
                                     lda, A + j0 + (j0 + jb) * lda, lda, T(1),
                                     A + (j0 + jb) + (j0 + jb) * lda, lda, s);
             }
         }
         if (rc != TMVX_OK) break;
-        // deferred interchanges on the columns outside the outer panel
-        for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
-            const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
-            rc = swap_cols(j0, jb, 0, J0, JE, n, nullptr);
+        if (side_pending) {
+            if (hipStreamWaitEvent(s, ev_side, 0) != hipSuccess) { rc = TMVX_ERR_HIP; break; }
+            side_pending = false;
         }
+        // Deferred interchanges on the columns outside the outer panel.  With
+        // a side stream to come, the caller's stream swaps only the next
+        // outer panel's columns; the side stream swaps the rest (the left
+        // L blocks and the far trailing columns) before its own work.
         const i64 nr = n - JE;
-        if (nr <= 0 || rc != TMVX_OK) continue;
-        // U12 = L11^-1 A12 for the outer block row, NB rows at a time
+        const i64 JE2 = (n - JE) < NBO ? n : JE + NBO;      // next outer panel [JE, JE2)
+        const bool has_side = nr > 0 && JE2 < n;
         for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
             const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
-            if ((rc = solve_rows(j0, jb, JE, nr)) != TMVX_OK) break;
-            const i64 below = JE - j0 - jb;
-            if (below > 0)
-                rc = gemm_launch<T>(0, 0, below, nr, jb, T(-1), A + (j0 + jb) + j0 * lda, lda,
-                                    A + j0 + JE * lda, lda, T(1), A + (j0 + jb) + JE * lda, lda,
-                                    s);
+            rc = has_side ? swap_cols(j0, jb, JE, JE2, JE2, JE2, nullptr, s)
+                          : swap_cols(j0, jb, 0, J0, JE, n, nullptr, s);
+        }
+        if (nr <= 0 || rc != TMVX_OK) continue;
+        // next panel's columns on the caller's stream
+        if ((rc = block_row_u12(J0, JE, JE, JE2, s)) != TMVX_OK) break;
+        rc = gemm_launch<T>(0, 0, nr, JE2 - JE, JE - J0, T(-1), A + JE + J0 * lda, lda,
+                            A + J0 + JE * lda, lda, T(1), A + JE + JE * lda, lda, s);
+        if (rc != TMVX_OK || JE2 >= n) continue;
+        // the rest of the trailing matrix on the side stream
+        if (!side) {
+            if (hipStreamCreateWithFlags(&side, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&ev_main, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&ev_side, hipEventDisableTiming) != hipSuccess) {
+                rc = TMVX_ERR_HIP;
+                break;
+            }
+        }
+        if (hipEventRecord(ev_main, s) != hipSuccess ||
+            hipStreamWaitEvent(side, ev_main, 0) != hipSuccess) {
+            rc = TMVX_ERR_HIP;
+            break;
+        }
+        for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
+            const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
+            rc = swap_cols(j0, jb, 0, J0, JE2, n, nullptr, side);
         }
         if (rc != TMVX_OK) break;
-        // A22 -= A21 * U12 with K = JE - J0
-        rc = gemm_launch<T>(0, 0, nr, nr, JE - J0, T(-1), A + JE + J0 * lda, lda,
-                            A + J0 + JE * lda, lda, T(1), A + JE + JE * lda, lda, s);
+        if ((rc = block_row_u12(J0, JE, JE2, n, side)) != TMVX_OK) break;
+        rc = gemm_launch<T>(0, 0, nr, n - JE2, JE - J0, T(-1), A + JE + J0 * lda, lda,
+                            A + J0 + JE2 * lda, lda, T(1), A + JE + JE2 * lda, lda, side);
+        if (rc != TMVX_OK) break;
+        if (hipEventRecord(ev_side, side) != hipSuccess) { rc = TMVX_ERR_HIP; break; }
+        side_pending = true;
+    }
+    // whatever happened above, the caller's stream waits for the side stream
+    if (side) {
+        if (hipEventRecord(ev_side, side) != hipSuccess ||
+            hipStreamWaitEvent(s, ev_side, 0) != hipSuccess)
+            rc = rc == TMVX_OK ? TMVX_ERR_HIP : rc;
     }
     int herr = 0;
     i64 fz = n + 1;
@@ -551,7 +610,13 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
         hipMemcpyAsync(&fz, wk.first_zero, sizeof(i64), hipMemcpyDeviceToHost, s) != hipSuccess)
         rc = TMVX_ERR_HIP;
     hipFreeAsync(ws, s);
-    if (hipStreamSynchronize(s) != hipSuccess) return TMVX_ERR_HIP;
+    const bool sync_ok = hipStreamSynchronize(s) == hipSuccess;
+    if (side) {
+        hipEventDestroy(ev_main);
+        hipEventDestroy(ev_side);
+        hipStreamDestroy(side);
+    }
+    if (!sync_ok) return TMVX_ERR_HIP;
     if (rc != TMVX_OK) return rc;
     if (herr) return TMVX_ERR_TIMEOUT;
     *info = fz <= n ? fz : 0;
