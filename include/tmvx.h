@@ -132,7 +132,10 @@ int tmvx_laswp_f32(int forward, int64_t ncols, float* A, int64_t lda,
  * whose solves are all in the matrix's own precision; the seam it extends
  * is x = b/A with divideUsing(LU), src/TMV_LUDiv.cpp:185-226).
  * A (n x n, FP64, not modified) is factorised in low precision on MFMA
- * (PA = LU, FP32 arithmetic), then for each of the nrhs columns
+ * (PA = LU with partial pivoting on an FP32 copy; lowprec selects the
+ * arithmetic of the trailing updates: FP32, or FP16 / BF16 with FP32
+ * accumulation and the U block row scaled by a power of two per outer
+ * panel), then for each of the nrhs columns
  *   r = b - A x  (FP64, on the GPU),  solve LU d = P r / s in FP32 with
  *   s = max|r|,  x += s d  (FP64)
  * until the normwise backward error ||r||_inf / (||A||_inf ||x||_inf +
@@ -141,7 +144,7 @@ int tmvx_laswp_f32(int forward, int64_t ncols, float* A, int64_t lda,
  * TMVX_ERR_NOT_CONVERGED (with X holding the best iterate) when the target
  * is not met.
  */
-enum tmvx_lowprec { TMVX_LOWPREC_F32 = 0 };
+enum tmvx_lowprec { TMVX_LOWPREC_F32 = 0, TMVX_LOWPREC_F16 = 1, TMVX_LOWPREC_BF16 = 2 };
 
 typedef struct tmvx_ir_info {
     int iterations;            /* refinement steps taken                  */
@@ -157,6 +160,18 @@ int tmvx_gesv_mixed_f64(int lowprec, int64_t n, int64_t nrhs, const double* A,
                         int64_t lda, const double* B, int64_t ldb, double* X,
                         int64_t ldx, double tol, int max_iter,
                         tmvx_ir_info* info, void* stream);
+
+/*
+ * C := alpha A B + beta C for FP32 column-major operands computed on FP16
+ * or BF16 MFMA (lowprec = TMVX_LOWPREC_F16 / _BF16) with FP32 accumulation:
+ * the trailing-update kernel of the low-precision factorisations.  A and B
+ * are rounded to the 16-bit format; scale_b != 0 first multiplies B by a
+ * power of two fitted to max |B| (and divides the product by it), so FP16
+ * keeps B's leading bits whatever its magnitude.  No counterpart in TMV.
+ */
+int tmvx_gemm_f32_lowp(int lowprec, int64_t m, int64_t n, int64_t k, float alpha,
+                       const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
+                       float* C, int64_t ldc, int scale_b, void* stream);
 
 /*
  * Adjoint of the solve X = A^-1 B with the factors of tmvx_getrf_*: given

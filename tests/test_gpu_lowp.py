@@ -1,0 +1,99 @@
+"""FP16 / BF16 trailing updates (tmvx_gemm_f32_lowp) and the mixed-precision
+solver with FP16 / BF16 factorisations, through the C ABI.
+
+Bar for the GEMM: elementwise |C - C_ref| <= 2 (2 u16 + k u32) (|A||B|)_ij
+(+ beta term) against a long-double product of the unrounded FP32 inputs,
+u16 the unit roundoff of the 16-bit format (FP16 2^-11, BF16 2^-8): each
+operand is rounded once, products of two 16-bit values are exact in FP32,
+sums accumulate in FP32.  Scaling keeps FP16 accurate for B far outside its
+range.  For the solver: backward error <= 1e-12 when it reports
+convergence (kappa = 1e2), and a clean not-converged report otherwise."""
+import numpy as np
+import pytest
+import torch
+
+import tmv_amd
+from checks import EPS32, backward_error
+from tmv_amd import rng
+
+pytestmark = pytest.mark.gpu
+SEED = 0x7A15EED
+U16 = {tmv_amd.LOWPREC_F16: 2.0 ** -11, tmv_amd.LOWPREC_BF16: 2.0 ** -8}
+NAME = {tmv_amd.LOWPREC_F16: "f16", tmv_amd.LOWPREC_BF16: "bf16"}
+
+
+def gemm_check(m, n, k, lp, scale_b=True, bscale=1.0, alpha=-1.0, beta=1.0, seed=0):
+    A = rng.uniform_matrix(SEED + seed, m, k).astype(np.float32)
+    B = (rng.uniform_matrix(SEED + seed + 1, k, n) * bscale).astype(np.float32)
+    C0 = rng.uniform_matrix(SEED + seed + 2, m, n).astype(np.float32)
+    dA = tmv_amd.colmajor(torch.from_numpy(A).cuda())
+    dB = tmv_amd.colmajor(torch.from_numpy(B).cuda())
+    dC = tmv_amd.colmajor(torch.from_numpy(C0).cuda())
+    tmv_amd.gemm_lowp(dA, dB, dC, alpha=alpha, beta=beta, lowprec=lp, scale_b=scale_b)
+    torch.cuda.synchronize()
+    C = dC.cpu().numpy().astype(np.float64)
+    ref = alpha * (A.astype(np.longdouble) @ B.astype(np.longdouble)) + beta * C0
+    mag = abs(alpha) * (np.abs(A).astype(np.float64) @ np.abs(B).astype(np.float64))
+    bound = 2 * (2 * U16[lp] + k * EPS32) * mag + 2 * EPS32 * abs(beta) * np.abs(C0) + 1e-30
+    err = np.abs(C - np.asarray(ref, np.float64))
+    return float((err / bound).max()), C
+
+
+@pytest.mark.parametrize("lp", [tmv_amd.LOWPREC_F16, tmv_amd.LOWPREC_BF16])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 5), (128, 128, 32), (300, 257, 256),
+                                   (1000, 700, 64), (513, 129, 300)])
+def test_gemm_lowp_bound(dev, lp, m, n, k):
+    print(f"gemm_lowp {NAME[lp]} {m}x{n}x{k}", flush=True)
+    worst, _ = gemm_check(m, n, k, lp, seed=m + n + k)
+    assert worst <= 1.0, worst
+
+
+@pytest.mark.parametrize("bscale", [1e6, 1e-6])
+def test_gemm_lowp_f16_scaling(dev, bscale):
+    """B far above FP16's range (1e6 > 65504) or deep in its subnormals:
+    the per-call power-of-two scale keeps the product accurate."""
+    worst, C = gemm_check(256, 256, 256, tmv_amd.LOWPREC_F16, scale_b=True, bscale=bscale,
+                          seed=7)
+    assert np.isfinite(C).all()
+    assert worst <= 1.0, worst
+
+
+def conditioned(n, kappa, seed):
+    g = np.random.default_rng(seed)
+    U, _ = np.linalg.qr(g.standard_normal((n, n)))
+    V, _ = np.linalg.qr(g.standard_normal((n, n)))
+    s = np.logspace(0, -np.log10(kappa), n)
+    return (U * s) @ V.T
+
+
+@pytest.mark.parametrize("lp", [tmv_amd.LOWPREC_F16, tmv_amd.LOWPREC_BF16])
+@pytest.mark.parametrize("n", [600, 2048])
+def test_mixed_lowp_converges(dev, lp, n):
+    print(f"mixed {NAME[lp]} n={n} kappa=1e2", flush=True)
+    A = conditioned(n, 1e2, SEED + n)
+    b = np.random.default_rng(n).standard_normal(n)
+    dA = tmv_amd.colmajor(torch.from_numpy(A).cuda())
+    db = torch.from_numpy(b).cuda()
+    x, info = tmv_amd.gesv_mixed(dA, db, tol=1e-12, max_iter=60, lowprec=lp)
+    torch.cuda.synchronize()
+    assert info["converged"] == 1 and info["status"] == tmv_amd.OK, info
+    assert info["backward_error"] <= 1e-12
+    assert backward_error(A, x.cpu().numpy(), b) <= 1e-12
+
+
+def test_mixed_lowp_reports_not_converged(dev):
+    """kappa = 1e6 is beyond what BF16 trailing updates plus classical
+    refinement can correct: the solver says so instead of returning a
+    wrong answer as converged."""
+    n = 1024
+    A = conditioned(n, 1e6, SEED + 9)
+    b = np.random.default_rng(9).standard_normal(n)
+    dA = tmv_amd.colmajor(torch.from_numpy(A).cuda())
+    db = torch.from_numpy(b).cuda()
+    x, info = tmv_amd.gesv_mixed(dA, db, tol=1e-12, max_iter=5, lowprec=tmv_amd.LOWPREC_BF16,
+                                 check=False)
+    torch.cuda.synchronize()
+    if info["status"] == tmv_amd.OK:
+        assert backward_error(A, x.cpu().numpy(), b) <= 1e-12
+    else:
+        assert info["status"] == tmv_amd.ERR_NOT_CONVERGED and info["converged"] == 0

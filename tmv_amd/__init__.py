@@ -65,6 +65,7 @@ _SIGS = {
     "tmvx_getrs_batched_f32": (_int, [_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "tmvx_gemm_batched_f64": (_int, [_i64, _vp, _vp, _vp, _d, _vp, _vp, _d, _vp, _vp]),
     "tmvx_gemm_batched_f32": (_int, [_i64, _vp, _vp, _vp, _f, _vp, _vp, _f, _vp, _vp]),
+    "tmvx_gemm_f32_lowp": (_int, [_int, _i64, _i64, _i64, _f, _vp, _i64, _vp, _i64, _f, _vp, _i64, _int, _vp]),
     "tmvx_getrf_batched_f64": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "tmvx_getrf_batched_f32": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "tmvx_gesv_adjoint_f64": (_int, [_i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _int, _vp]),
@@ -270,6 +271,8 @@ class IrInfo(ctypes.Structure):
 
 
 LOWPREC_F32 = 0
+LOWPREC_F16 = 1
+LOWPREC_BF16 = 2
 
 
 def gesv_mixed(A, B, X=None, tol=1e-12, max_iter=30, lowprec=LOWPREC_F32, check=True,
@@ -500,3 +503,21 @@ def gesv_adjoint_batched(fact: Batch, X: Batch, G: Batch, gradB: Batch, gradA: B
                   fact.pP.data_ptr(), X.pB.data_ptr(), G.pB.data_ptr(), gradB.pB.data_ptr(),
                   gradA.pA.data_ptr(), int(bool(accumulate)), _stream(stream)),
                "gesv_adjoint_batched")
+
+
+def gemm_lowp(A, B, C=None, alpha=1.0, beta=0.0, lowprec=LOWPREC_F16, scale_b=True, stream=None):
+    """C := alpha A B + beta C for column-major float32 tensors on FP16 / BF16
+    MFMA with FP32 accumulation (tmvx_gemm_f32_lowp)."""
+    torch = _torch()
+    m, k = A.shape
+    n = B.shape[1]
+    lda = _colmajor_ld(A, "A")
+    ldb = _colmajor_ld(B, "B")
+    if C is None:
+        C = empty_colmajor(m, n, torch.float32, A.device)
+        beta = 0.0
+    ldc = _colmajor_ld(C, "C")
+    _check(lib().tmvx_gemm_f32_lowp(lowprec, m, n, k, alpha, A.data_ptr(), lda, B.data_ptr(),
+                                    ldb, beta, C.data_ptr(), ldc, int(bool(scale_b)),
+                                    _stream(stream)), "gemm_lowp")
+    return C

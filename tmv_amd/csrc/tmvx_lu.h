@@ -8,8 +8,18 @@ namespace tmvx {
 // In-place PA = LU (pivots 0-based interchanges).  When perm != nullptr it
 // also receives the composed row permutation: row i of PA is row perm[i] of
 // A.  Synchronises; *info = 1-based first zero pivot or 0.
+// lowprec (FP32 only): TMVX_LOWPREC_F16 / _BF16 run the trailing updates
+// on FP16 / BF16 MFMA with per-block-row scaling (gemm_lowp_launch).
 template <typename T>
-int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s);
+int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s,
+               int lowprec = TMVX_LOWPREC_F32);
+
+// C := beta C + alpha A B on FP16 / BF16 MFMA, FP32 operands and result;
+// with bmax_word != nullptr, B is scaled by a power of two fitted to
+// max |B| (computed into *bmax_word first) and the result unscaled.
+int gemm_lowp_launch(int lowprec, i64 m, i64 n, i64 k, float alpha, const float* A, i64 lda,
+                     const float* B, i64 ldb, float beta, float* C, i64 ldc, unsigned* bmax_word,
+                     hipStream_t s);
 
 // Interchange list -> composed permutation (row i of PA = row perm[i] of A).
 int piv_to_perm(i64 n, const i64* piv, i64* perm, hipStream_t s);

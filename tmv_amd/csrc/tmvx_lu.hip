@@ -198,25 +198,54 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
         }
         PANEL_TRACE(k, 2);
         // -- receive every slab's words once they carry this column's tag --
+        //    Each thread issues all of its loads, then re-polls only the
+        //    words that are still stale: one round trip per poll, not one
+        //    per word.
         {
             const int nrow = G * SL::DIAG;               // candidate-row words
             const int total = nrow + SL::DIAG + G * (WPV + 1);
+            constexpr int CW = sizeof(T) == 8 ? 4 : 8;   // words per thread per poll
+            auto src_of = [&](int e) -> const unsigned long long* {
+                if (e < nrow)
+                    return wk.slots + ((i64)par * G + e / SL::DIAG) * SL::SIZE + SL::ROW +
+                           e % SL::DIAG;
+                if (e < nrow + SL::DIAG)
+                    return wk.slots + (i64)par * G * SL::SIZE + SL::DIAG + (e - nrow);
+                const int f = e - nrow - SL::DIAG, q = f / (WPV + 1), h = f % (WPV + 1);
+                return wk.slots + ((i64)par * G + q) * SL::SIZE + SL::VAL + h;
+            };
             unsigned spins = 0;
             bool ok = true;
-            for (int e = t; e < total && ok; e += NTH) {
-                const unsigned long long* src;
-                if (e < nrow) {
-                    src = wk.slots + ((i64)par * G + e / SL::DIAG) * SL::SIZE + SL::ROW + e % SL::DIAG;
-                } else if (e < nrow + SL::DIAG) {
-                    src = wk.slots + (i64)par * G * SL::SIZE + SL::DIAG + (e - nrow);
-                } else {
-                    const int f = e - nrow - SL::DIAG, q = f / (WPV + 1), h = f % (WPV + 1);
-                    src = wk.slots + ((i64)par * G + q) * SL::SIZE + SL::VAL + h;
-                }
-                unsigned long long word;
-                while (((word = cload(src)) & ~0xffffffffull) != tag) {
-                    if ((++spins & 255) == 0 &&
-                        (spins > SPIN_LIMIT ||
+            for (int e0 = t; e0 < total && ok; e0 += CW * NTH) {
+                unsigned pend = 0;                        // bit i: word e0 + i*NTH stale
+                static_for<0, CW>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    if (e0 + i * NTH < total) pend |= 1u << i;
+                });
+                while (pend) {
+                    unsigned long long wv[CW];
+                    static_for<0, CW>([&](auto ic) {
+                        constexpr int i = decltype(ic)::value;
+                        wv[i] = (pend >> i) & 1u ? cload(src_of(e0 + i * NTH)) : 0ull;
+                    });
+                    static_for<0, CW>([&](auto ic) {
+                        constexpr int i = decltype(ic)::value;
+                        if (((pend >> i) & 1u) && (wv[i] & ~0xffffffffull) == tag) {
+                            pend &= ~(1u << i);
+                            const int e = e0 + i * NTH;
+                            const unsigned pay = (unsigned)wv[i];
+                            if (e < nrow) s_cand[e / SL::DIAG][e % SL::DIAG] = pay;
+                            else if (e < nrow + SL::DIAG) s_diag[e - nrow] = pay;
+                            else {
+                                const int f = e - nrow - SL::DIAG, q = f / (WPV + 1),
+                                          h = f % (WPV + 1);
+                                if (h < WPV) s_cval[q][h] = pay; else s_cidx[q] = (int)pay;
+                            }
+                        }
+                    });
+                    if (!pend) break;
+                    if (++spins > SPIN_LIMIT ||
+                        ((spins & 255) == 0 &&
                          __hip_atomic_load(wk.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                         __hip_atomic_store(wk.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         s_fail = 1;
@@ -224,14 +253,6 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
-                }
-                if (!ok) break;
-                const unsigned pay = (unsigned)word;
-                if (e < nrow) s_cand[e / SL::DIAG][e % SL::DIAG] = pay;
-                else if (e < nrow + SL::DIAG) s_diag[e - nrow] = pay;
-                else {
-                    const int f = e - nrow - SL::DIAG, q = f / (WPV + 1), h = f % (WPV + 1);
-                    if (h < WPV) s_cval[q][h] = pay; else s_cidx[q] = (int)pay;
                 }
             }
         }
@@ -453,7 +474,7 @@ int zero_bytes(void* p, size_t bytes, hipStream_t s)
 }
 
 template <typename T>
-int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s)
+int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s, int lowprec)
 {
     if (n < 0 || lda < (n > 0 ? n : 1) || !info) return TMVX_ERR_ARG;
     *info = 0;
@@ -473,6 +494,20 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
     char* tail = ws + szslots;
     wk.err = reinterpret_cast<int*>(tail);
     wk.first_zero = reinterpret_cast<i64*>(tail + 64);
+    unsigned* bmax_main = reinterpret_cast<unsigned*>(tail + 128);   // low-precision scaling
+    unsigned* bmax_side = reinterpret_cast<unsigned*>(tail + 192);
+    const bool lowp = sizeof(T) == 4 && lowprec != TMVX_LOWPREC_F32;
+    // Trailing update A22 -= A21 U12 (K = JE - J0): FP32/FP64 MFMA, or
+    // FP16/BF16 MFMA with the U12 block scaled per block row.
+    auto trailing = [&](i64 rows, i64 cols, i64 K, const T* A21, const T* U12, T* A22,
+                        unsigned* word, hipStream_t st) -> int {
+        if constexpr (sizeof(T) == 4) {
+            if (lowp)
+                return gemm_lowp_launch(lowprec, rows, cols, K, -1.0f, A21, lda, U12, lda, 1.0f,
+                                        A22, lda, word, st);
+        }
+        return gemm_launch<T>(0, 0, rows, cols, K, T(-1), A21, lda, U12, lda, T(1), A22, lda, st);
+    };
 
     int rc = TMVX_OK;
     {
@@ -569,8 +604,8 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
         if (nr <= 0 || rc != TMVX_OK) continue;
         // next panel's columns on the caller's stream
         if ((rc = block_row_u12(J0, JE, JE, JE2, s)) != TMVX_OK) break;
-        rc = gemm_launch<T>(0, 0, nr, JE2 - JE, JE - J0, T(-1), A + JE + J0 * lda, lda,
-                            A + J0 + JE * lda, lda, T(1), A + JE + JE * lda, lda, s);
+        rc = trailing(nr, JE2 - JE, JE - J0, A + JE + J0 * lda, A + J0 + JE * lda,
+                      A + JE + JE * lda, bmax_main, s);
         if (rc != TMVX_OK || JE2 >= n) continue;
         // the rest of the trailing matrix on the side stream
         if (!side) {
@@ -592,8 +627,8 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
         }
         if (rc != TMVX_OK) break;
         if ((rc = block_row_u12(J0, JE, JE2, n, side)) != TMVX_OK) break;
-        rc = gemm_launch<T>(0, 0, nr, n - JE2, JE - J0, T(-1), A + JE + J0 * lda, lda,
-                            A + J0 + JE2 * lda, lda, T(1), A + JE + JE2 * lda, lda, side);
+        rc = trailing(nr, n - JE2, JE - J0, A + JE + J0 * lda, A + J0 + JE2 * lda,
+                      A + JE + JE2 * lda, bmax_side, side);
         if (rc != TMVX_OK) break;
         if (hipEventRecord(ev_side, side) != hipSuccess) { rc = TMVX_ERR_HIP; break; }
         side_pending = true;
@@ -656,8 +691,8 @@ int piv_to_perm(i64 n, const i64* piv, i64* perm, hipStream_t s)
     return hip_status(hipGetLastError());
 }
 
-template int getrf_impl<double>(i64, double*, i64, i64*, i64*, i64*, hipStream_t);
-template int getrf_impl<float>(i64, float*, i64, i64*, i64*, i64*, hipStream_t);
+template int getrf_impl<double>(i64, double*, i64, i64*, i64*, i64*, hipStream_t, int);
+template int getrf_impl<float>(i64, float*, i64, i64*, i64*, i64*, hipStream_t, int);
 
 } // namespace tmvx
 

@@ -163,7 +163,7 @@ static int launch_residual(i64 n, i64 nr, const double* A, i64 lda, const double
     return TMVX_OK;
 }
 
-int gesv_mixed_f32(i64 n, i64 nrhs, const double* A, i64 lda, const double* B, i64 ldb,
+int gesv_mixed_f32(int lowprec, i64 n, i64 nrhs, const double* A, i64 lda, const double* B, i64 ldb,
                    double* X, i64 ldx, double tol, int max_iter, tmvx_ir_info* info,
                    hipStream_t s)
 {
@@ -229,7 +229,7 @@ int gesv_mixed_f32(i64 n, i64 nrhs, const double* A, i64 lda, const double* B, i
     }
     // 2. FP32 LU with partial pivoting (synchronises).
     i64 finfo = 0;
-    if (rc == TMVX_OK) rc = getrf_impl<float>(n, A32, n, piv, perm, &finfo, s);
+    if (rc == TMVX_OK) rc = getrf_impl<float>(n, A32, n, piv, perm, &finfo, s, lowprec);
     (void)hipEventRecord(e1, s);
     info->factor_info = (int)finfo;
     if (rc == TMVX_OK && finfo != 0) rc = TMVX_ERR_SINGULAR;
@@ -305,7 +305,9 @@ extern "C" int tmvx_gesv_mixed_f64(int lowprec, int64_t n, int64_t nrhs, const d
                                    int64_t ldx, double tol, int max_iter, tmvx_ir_info* info,
                                    void* stream)
 {
-    if (lowprec != TMVX_LOWPREC_F32) return TMVX_ERR_UNSUPPORTED;
-    return tmvx::gesv_mixed_f32(n, nrhs, A, lda, B, ldb, X, ldx, tol, max_iter, info,
+    if (lowprec != TMVX_LOWPREC_F32 && lowprec != TMVX_LOWPREC_F16 &&
+        lowprec != TMVX_LOWPREC_BF16)
+        return TMVX_ERR_UNSUPPORTED;
+    return tmvx::gesv_mixed_f32(lowprec, n, nrhs, A, lda, B, ldb, X, ldx, tol, max_iter, info,
                                 (hipStream_t)stream);
 }
