@@ -14,6 +14,8 @@ solve (the HPL-MxP convention), inputs resident in HBM.
 
 Also measured in the same run (outside the K timed steps):
   - the plain FP64 GPU LU solve (getrf + getrs) on the same A and b;
+  - the same solver with FP16 / BF16 (and FP32) trailing updates on a
+    kappa = 1e2 matrix of the same construction;
   - DGEMM M = N = K = 8192 in the CCC, CRC and RCC operand layouts; the
     CCC kernel is the `roofline` entry (FP64 MFMA, 78.6 TFLOP/s peak),
     timed with HIP events on the stream it runs on;
@@ -302,6 +304,7 @@ def main():
     ap.add_argument("--gemm-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-lowp", dest="lowp", action="store_false")
     ap.add_argument("--only-batched", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -346,6 +349,27 @@ def main():
     flops = 2.0 / 3.0 * n ** 3 + 2.0 * n * n * nrhs
     value = flops / t_mixed / 1e12
     be_mixed = backward_error_gpu(A, X, b, torch)
+
+    # ---- FP16 / BF16 factorisations (kappa = 1e2, where classical
+    #      refinement from a 16-bit trailing update converges) --------------
+    lowp = {}
+    if a.lowp:
+        A2 = conditioned_gpu(n, 1e2, SEED + 14, torch, tmv_amd)
+        for name, lp in (("f16", tmv_amd.LOWPREC_F16), ("bf16", tmv_amd.LOWPREC_BF16),
+                         ("f32", tmv_amd.LOWPREC_F32)):
+            tmv_amd.gesv_mixed(A2, b, tol=1e-12, max_iter=60, lowprec=lp, check=False)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            Xl, inf = tmv_amd.gesv_mixed(A2, b, tol=1e-12, max_iter=60, lowprec=lp, check=False)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t1
+            lowp[name] = {"ms": round(dt * 1e3, 3), "tflops": round(flops / dt / 1e12, 3),
+                          "iterations": inf["iterations"], "converged": inf["converged"],
+                          "backward_error": backward_error_gpu(A2, Xl, b, torch),
+                          "factor_ms": round(inf["factor_ms"], 3),
+                          "refine_ms": round(inf["refine_ms"], 3)}
+        del A2, Xl
+        torch.cuda.empty_cache()
 
     # ---- plain FP64 GPU LU solve on the same A, b ---------------------------
     t64 = []
@@ -433,6 +457,7 @@ def main():
                           "tflops": round(flops / t_fp64 / 1e12, 3),
                           "backward_error": be64},
         "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
+        "mixed_kappa1e2": lowp or None,
         "dgemm_8192": gemm,
         "batched_20000": batched,
         "adjoint_512x256": adjoint,

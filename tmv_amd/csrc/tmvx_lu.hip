@@ -126,7 +126,7 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
     __shared__ unsigned s_diag[NB * WPV];
     __shared__ unsigned s_cval[PANEL_GMAX][WPV];
     __shared__ int s_cidx[PANEL_GMAX];
-    __shared__ T s_out[2][NB];                 // rows to publish (cand, diag)
+    __shared__ __attribute__((aligned(16))) T s_out[2][NB];   // rows to publish (cand, diag)
     __shared__ __attribute__((aligned(16))) T s_prow[NB];   // winning row
     __shared__ __attribute__((aligned(16))) T s_orow[NB];   // row k before the swap
     __shared__ T s_wv[NW];
@@ -166,23 +166,32 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
         wave_argmax(v, vi);
         if (lane == 0) { s_wv[wave] = v; s_wi[wave] = vi; }
         __syncthreads();
-        if (t == 0) {
-            T bv = s_wv[0]; int bi = s_wi[0];
-            for (int q = 1; q < NW; ++q)
-                if (better(s_wv[q], s_wi[q], bv, bi)) { bv = s_wv[q]; bi = s_wi[q]; }
-            s_bv = bv;
-            s_p = bi;
+        if (wave == 0) {                 // across waves: one more DPP argmax
+            T bv = lane < NW ? s_wv[lane] : T(-1);
+            int bi = lane < NW ? s_wi[lane] : INT_MAX;
+            wave_argmax(bv, bi);
+            if (lane == 0) { s_bv = bv; s_p = bi; }
         }
         __syncthreads();
         PANEL_TRACE(k, 1);
         // -- publish the candidate row and (slab 0) row k as tagged words --
-        if (own && r == s_p) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) s_out[0][c] = row[c];
+        typedef typename Vec16<T>::type VT;
+        constexpr int VN = Vec16<T>::n;
+        if (own && r == s_p) {          // 16-byte LDS stores
+            static_for<0, NB / VN>([&](auto cc) {
+                constexpr int c = decltype(cc)::value * VN;
+                VT x;
+                static_for<0, VN>([&](auto uc) { x[decltype(uc)::value] = row[c + decltype(uc)::value]; });
+                *reinterpret_cast<VT*>(&s_out[0][c]) = x;
+            });
         }
         if (own && r == k) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) s_out[1][c] = row[c];
+            static_for<0, NB / VN>([&](auto cc) {
+                constexpr int c = decltype(cc)::value * VN;
+                VT x;
+                static_for<0, VN>([&](auto uc) { x[decltype(uc)::value] = row[c + decltype(uc)::value]; });
+                *reinterpret_cast<VT*>(&s_out[1][c]) = x;
+            });
         }
         __syncthreads();
         for (int e = t; e < SL::SIZE; e += NTH) {
@@ -260,17 +269,23 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
         __syncthreads();
         PANEL_TRACE(k, 4);
         if (s_fail) { alive = false; break; }
-        // -- global winner (identical in every workgroup) -----------------
-        if (t == 0) {
-            T bv = T(-1); int bi = INT_MAX, bw = 0;
-            for (int q = 0; q < G; ++q) {
-                const T qv = from_halves<T>(s_cval[q]);
-                if (better(qv, s_cidx[q], bv, bi)) { bv = qv; bi = s_cidx[q]; bw = q; }
+        // -- global winner (identical in every workgroup): lane q of wave 0
+        //    holds slab q's candidate, one DPP argmax picks the best --------
+        if (wave == 0) {
+            T bv = T(-1);
+            int bi = INT_MAX;
+            if (lane < G) { bv = from_halves<T>(s_cval[lane]); bi = s_cidx[lane]; }
+            const T myv = bv;
+            const int myi = bi;
+            wave_argmax(bv, bi);
+            // the winning slab: lowest lane holding the winning candidate
+            const unsigned long long hit = __ballot(lane < G && myv == bv && myi == bi);
+            if (lane == 0) {
+                const bool zero = !(bv >= tiny<T>());   // TMV_Underflow
+                s_zero = zero ? 1 : 0;
+                s_p = zero ? k : bi;
+                s_win = hit ? (int)__builtin_ctzll(hit) : 0;
             }
-            const bool zero = !(bv >= tiny<T>());   // TMV_Underflow
-            s_zero = zero ? 1 : 0;
-            s_p = zero ? k : bi;
-            s_win = bw;
         }
         __syncthreads();
         if (t < NB) s_prow[t] = from_halves<T>(&s_cand[s_win][t * WPV]);
@@ -295,13 +310,28 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
                 for (int c = 0; c < NB; ++c) row[c] = s_prow[c];
             }
             if (own && r > k) {
-                const T inv = T(1) / s_prow[0];
-                const T l = row[0] * inv;
-                row[0] = l;
                 // rotated columns 1 .. NB-1-k are the panel columns k+1 .. NB-1
+                if constexpr (sizeof(T) == 4) {
+                    T pr[NB];                   // pivot row, 16-byte LDS loads
+                    static_for<0, NB / VN>([&](auto cc) {
+                        constexpr int c = decltype(cc)::value * VN;
+                        const VT x = *reinterpret_cast<const VT*>(&s_prow[c]);
+                        static_for<0, VN>([&](auto uc) {
+                            pr[c + decltype(uc)::value] = x[decltype(uc)::value];
+                        });
+                    });
+                    const T l = row[0] * (T(1) / pr[0]);
+                    row[0] = l;
 #pragma unroll
-                for (int c = 1; c < NB; ++c)
-                    if (c < NB - k) row[c] -= l * s_prow[c];
+                    for (int c = 1; c < NB; ++c)
+                        if (c < NB - k) row[c] -= l * pr[c];
+                } else {                        // (registers: the row alone is 128)
+                    const T l = row[0] * (T(1) / s_prow[0]);
+                    row[0] = l;
+#pragma unroll
+                    for (int c = 1; c < NB; ++c)
+                        if (c < NB - k) row[c] -= l * s_prow[c];
+                }
             }
             if (w == 0 && t == 0) piv[j0 + k] = j0 + p;
         }
