@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <thread>
 #include <vector>
 
 namespace tmvx {
@@ -185,16 +186,41 @@ static int getrf_batched_impl(int64_t count, const int32_t* n_host, T* const* A,
         hipFree(dptr);
         hipFree(dinfo);
     }
-    for (size_t k = 0; k < big.size() && rc == TMVX_OK; ++k) {
-        const int i = big[k];
-        i64 inf = 0;
-        rc = getrf_impl<T>(n_host[i], hA[i], n_host[i], (i64*)hP[i], nullptr, &inf, s);
-        const int32_t iv = (int32_t)inf;
-        if (rc == TMVX_OK &&
-            (hipMemcpyAsync(info + i, &iv, sizeof(int32_t), hipMemcpyHostToDevice, s) !=
-                 hipSuccess ||
-             hipStreamSynchronize(s) != hipSuccess))
-            rc = TMVX_ERR_HIP;
+    // Systems above the batched kernel's limit: the blocked LU, one system
+    // per call, from up to 16 host threads with a stream each (every call
+    // synchronises its own stream, so the calls overlap on the GPU).
+    if (rc == TMVX_OK && !big.empty()) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return TMVX_ERR_HIP;
+        const int nth = (int)std::min<size_t>(big.size(), 16);
+        std::vector<int> trc(nth, TMVX_OK);
+        std::vector<int32_t> hinfo(big.size(), 0);
+        auto work = [&](int tid) {
+            hipStream_t st = nullptr;
+            if (hipSetDevice(dev) != hipSuccess ||
+                hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+                trc[tid] = TMVX_ERR_HIP;
+                return;
+            }
+            for (size_t k = tid; k < big.size() && trc[tid] == TMVX_OK; k += nth) {
+                const int i = big[k];
+                i64 inf = 0;
+                trc[tid] = getrf_impl<T>(n_host[i], hA[i], n_host[i], (i64*)hP[i], nullptr, &inf, st);
+                hinfo[k] = (int32_t)inf;
+            }
+            hipStreamDestroy(st);
+        };
+        std::vector<std::thread> pool;
+        for (int tid = 1; tid < nth; ++tid) pool.emplace_back(work, tid);
+        work(0);
+        for (auto& th : pool) th.join();
+        for (int tid = 0; tid < nth; ++tid)
+            if (trc[tid] != TMVX_OK) rc = trc[tid];
+        for (size_t k = 0; k < big.size() && rc == TMVX_OK; ++k)
+            if (hipMemcpyAsync(info + big[k], &hinfo[k], sizeof(int32_t), hipMemcpyHostToDevice,
+                               s) != hipSuccess)
+                rc = TMVX_ERR_HIP;
+        if (rc == TMVX_OK && hipStreamSynchronize(s) != hipSuccess) rc = TMVX_ERR_HIP;
     }
     return rc;
 }
