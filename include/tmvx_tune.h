@@ -29,6 +29,14 @@ int tmvx_gemm_cfg_f32(int cfg, int layout_a, int layout_b, int64_t m, int64_t n,
                       const float* B, int64_t ldb, float beta, float* C,
                       int64_t ldc, void* stream);
 
+/*
+ * PA = LU in FP32 with tournament pivoting (the mixed-precision solver's
+ * factorisation; not TMV's pivot sequence).  Arguments as tmvx_getrf_f32.
+ * No counterpart in TMV; exposed for tests and experiments.
+ */
+int tmvx_getrf_f32_tournament(int64_t n, float* A, int64_t lda, int64_t* piv,
+                              int64_t* info, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,7 @@ _SIGS = {
     "tmvx_getrs_batched_f32": (_int, [_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "tmvx_gemm_batched_f64": (_int, [_i64, _vp, _vp, _vp, _d, _vp, _vp, _d, _vp, _vp]),
     "tmvx_gemm_batched_f32": (_int, [_i64, _vp, _vp, _vp, _f, _vp, _vp, _f, _vp, _vp]),
+    "tmvx_getrf_f32_tournament": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
     "tmvx_gemm_f32_lowp": (_int, [_int, _i64, _i64, _i64, _f, _vp, _i64, _vp, _i64, _f, _vp, _i64, _int, _vp]),
     "tmvx_getrf_batched_f64": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "tmvx_getrf_batched_f32": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
@@ -222,6 +223,21 @@ def getrf(A, stream=None):
     fn = getattr(lib(), "tmvx_getrf_" + _suffix(A.dtype))
     _check(fn(n, A.data_ptr(), lda, piv.data_ptr(), ctypes.byref(info), _stream(stream)),
            "getrf")
+    return piv, int(info.value)
+
+
+def getrf_tournament(A, stream=None):
+    """FP32 PA = LU with tournament pivoting (the mixed solver's panel
+    factorisation).  Returns (piv, info) like getrf."""
+    torch = _torch()
+    n = A.shape[0]
+    lda = _colmajor_ld(A, "A")
+    if A.dtype != torch.float32:
+        raise TypeError("float32 only")
+    piv = torch.empty(n, dtype=torch.int64, device=A.device)
+    info = _i64(0)
+    _check(lib().tmvx_getrf_f32_tournament(n, A.data_ptr(), lda, piv.data_ptr(),
+                                           ctypes.byref(info), _stream(stream)), "getrf_tournament")
     return piv, int(info.value)
 
 

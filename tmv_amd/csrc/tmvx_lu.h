@@ -10,9 +10,21 @@ namespace tmvx {
 // A.  Synchronises; *info = 1-based first zero pivot or 0.
 // lowprec (FP32 only): TMVX_LOWPREC_F16 / _BF16 run the trailing updates
 // on FP16 / BF16 MFMA with per-block-row scaling (gemm_lowp_launch).
+// tournament != 0 (FP32 only): tournament pivoting (tmvx_tslu.hip) instead
+// of partial pivoting -- for the mixed-precision solver, not for TMV parity.
 template <typename T>
 int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s,
-               int lowprec = TMVX_LOWPREC_F32);
+               int lowprec = TMVX_LOWPREC_F32, int tournament = 0);
+
+// Tournament pivoting pieces (tmvx_tslu.hip): selection tree + interchange
+// list for panel [j0, j0+jb), then the factorisation of the panel once its
+// interchanges are applied to its columns.
+i64 tslu_ws_ints(i64 m);
+template <typename T>
+int tslu_panel(T* A, i64 lda, i64 n, i64 j0, int jb, i64* piv, i64* first_zero, int* ws,
+               hipStream_t s);
+template <typename T>
+int tslu_factor(T* A, i64 lda, i64 n, i64 j0, int jb, i64* first_zero, hipStream_t s);
 
 // C := beta C + alpha A B on FP16 / BF16 MFMA, FP32 operands and result;
 // with bmax_word != nullptr, B is scaled by a power of two fitted to

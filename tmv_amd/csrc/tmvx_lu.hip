@@ -504,7 +504,8 @@ int zero_bytes(void* p, size_t bytes, hipStream_t s)
 }
 
 template <typename T>
-int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s, int lowprec)
+int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s, int lowprec,
+               int tournament)
 {
     if (n < 0 || lda < (n > 0 ? n : 1) || !info) return TMVX_ERR_ARG;
     *info = 0;
@@ -516,7 +517,9 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
 
     const i64 nslot = 2 * (i64)Gmax * SlotLayout<T, NB>::SIZE;
     const size_t szslots = sizeof(unsigned long long) * (size_t)nslot;
-    const size_t total = szslots + 256;
+    const bool tslu = sizeof(T) == 4 && tournament != 0;
+    const size_t szts = tslu ? sizeof(int) * (size_t)tslu_ws_ints(n) : 0;
+    const size_t total = szslots + 256 + szts;
     char* ws = nullptr;
     if (hipMallocAsync((void**)&ws, total, s) != hipSuccess) return TMVX_ERR_ALLOC;
     PanelWork wk;
@@ -524,6 +527,7 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
     char* tail = ws + szslots;
     wk.err = reinterpret_cast<int*>(tail);
     wk.first_zero = reinterpret_cast<i64*>(tail + 64);
+    int* ts_ws = tslu ? reinterpret_cast<int*>(tail + 256) : nullptr;
     unsigned* bmax_main = reinterpret_cast<unsigned*>(tail + 128);   // low-precision scaling
     unsigned* bmax_side = reinterpret_cast<unsigned*>(tail + 192);
     const bool lowp = sizeof(T) == 4 && lowprec != TMVX_LOWPREC_F32;
@@ -600,10 +604,24 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
         const i64 JE = (n - J0) < NBO ? n : J0 + NBO;      // outer panel [J0, JE)
         for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
             const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
-            const int G = (int)((n - j0 + NTH - 1) / NTH);
-            hipLaunchKernelGGL((panel_kernel<T, NB, NTH>), dim3(G), dim3(NTH), 0, s,
-                               A, lda, n, j0, jb, piv, wk);
-            if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
+            if constexpr (sizeof(T) == 4) {
+                if (tslu) {
+                    // tournament pivots, their interchanges on the panel's
+                    // own columns, then the panel's L and U
+                    if ((rc = tslu_panel<T>(A, lda, n, j0, jb, piv, wk.first_zero, ts_ws, s)) !=
+                            TMVX_OK ||
+                        (rc = swap_cols(j0, jb, j0, j0 + jb, j0 + jb, j0 + jb, nullptr, s)) !=
+                            TMVX_OK ||
+                        (rc = tslu_factor<T>(A, lda, n, j0, jb, wk.first_zero, s)) != TMVX_OK)
+                        break;
+                }
+            }
+            if (!tslu) {
+                const int G = (int)((n - j0 + NTH - 1) / NTH);
+                hipLaunchKernelGGL((panel_kernel<T, NB, NTH>), dim3(G), dim3(NTH), 0, s,
+                                   A, lda, n, j0, jb, piv, wk);
+                if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
+            }
             // inside the outer panel only; the rest is deferred
             if ((rc = swap_cols(j0, jb, J0, j0, j0 + jb, JE, perm, s)) != TMVX_OK) break;
             const i64 nr = JE - j0 - jb;
@@ -721,8 +739,8 @@ int piv_to_perm(i64 n, const i64* piv, i64* perm, hipStream_t s)
     return hip_status(hipGetLastError());
 }
 
-template int getrf_impl<double>(i64, double*, i64, i64*, i64*, i64*, hipStream_t, int);
-template int getrf_impl<float>(i64, float*, i64, i64*, i64*, i64*, hipStream_t, int);
+template int getrf_impl<double>(i64, double*, i64, i64*, i64*, i64*, hipStream_t, int, int);
+template int getrf_impl<float>(i64, float*, i64, i64*, i64*, i64*, hipStream_t, int, int);
 
 } // namespace tmvx
 
@@ -740,5 +758,12 @@ int tmvx_getrf_f64(int64_t n, double* A, int64_t lda, int64_t* piv, int64_t* inf
 { return tmvx::getrf_impl<double>(n, A, lda, piv, nullptr, info, (hipStream_t)stream); }
 int tmvx_getrf_f32(int64_t n, float* A, int64_t lda, int64_t* piv, int64_t* info, void* stream)
 { return tmvx::getrf_impl<float>(n, A, lda, piv, nullptr, info, (hipStream_t)stream); }
+
+int tmvx_getrf_f32_tournament(int64_t n, float* A, int64_t lda, int64_t* piv, int64_t* info,
+                              void* stream)
+{
+    return tmvx::getrf_impl<float>(n, A, lda, piv, nullptr, info, (hipStream_t)stream,
+                                   TMVX_LOWPREC_F32, 1);
+}
 
 } // extern "C"

@@ -229,7 +229,10 @@ int gesv_mixed_f32(int lowprec, i64 n, i64 nrhs, const double* A, i64 lda, const
     }
     // 2. FP32 LU with partial pivoting (synchronises).
     i64 finfo = 0;
-    if (rc == TMVX_OK) rc = getrf_impl<float>(n, A32, n, piv, perm, &finfo, s, lowprec);
+    // partial pivoting (the tournament-pivoted panel of tmvx_tslu.hip is
+    // slower on this GPU so far: its local selection costs as much per
+    // column as the partial-pivoting panel's exchange and local work)
+    if (rc == TMVX_OK) rc = getrf_impl<float>(n, A32, n, piv, perm, &finfo, s, lowprec, 0);
     (void)hipEventRecord(e1, s);
     info->factor_info = (int)finfo;
     if (rc == TMVX_OK && finfo != 0) rc = TMVX_ERR_SINGULAR;
