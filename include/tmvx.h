@@ -104,6 +104,36 @@ int tmvx_getrs_f32(int trans, int64_t n, int64_t nrhs, const float* LU,
                    int check_singular, void* stream);
 
 /*
+ * X := A^-1 from the factors of tmvx_getrf_* (X n x n column-major, not
+ * aliasing LU).  The same matrix TMV's LU_Inverse returns (U^-1 L^-1 with
+ * the columns reverse-permuted by piv); formed here as the solve of
+ * A X = I with all n columns at once.  A zero U diagonal reports
+ * TMVX_ERR_SINGULAR.  n <= 65535.
+ * Replaces: tmv::LU_Inverse (src/TMV_LUDiv.h:38, src/TMV_LUInverse.cpp:
+ * 235-275) behind LUDiv::doMakeInverse (src/TMV_LUD.cpp:158-169), i.e.
+ * m.inverse() with LU division.
+ */
+int tmvx_getri_f64(int64_t n, const double* LU, int64_t lda, const int64_t* piv,
+                   double* X, int64_t ldx, void* stream);
+int tmvx_getri_f32(int64_t n, const float* LU, int64_t lda, const int64_t* piv,
+                   float* X, int64_t ldx, void* stream);
+
+/*
+ * log|det A| and sign(det A) from the factors of tmvx_getrf_*, written to
+ * host memory (the call synchronises the stream): logdet = sum log|U_ii|
+ * (in FP64), sign = det(P) * prod sign(U_ii), det(P) = (-1)^(number of i
+ * with piv[i] != i).  A zero diagonal gives logdet = -inf and sign = 0;
+ * det A = sign * exp(logdet), or 0 when sign is 0.
+ * Replaces: LUDiv::det / LUDiv::logDet (src/TMV_LUD.cpp:133-156) with
+ * DiagMatrix::logDet (src/TMV_DiagMatrix.cpp:59-90) and Permutation::det
+ * (include/tmv/TMV_Permutation.h:273-278).
+ */
+int tmvx_logdet_f64(int64_t n, const double* LU, int64_t lda, const int64_t* piv,
+                    double* logdet, double* sign, void* stream);
+int tmvx_logdet_f32(int64_t n, const float* LU, int64_t lda, const int64_t* piv,
+                    double* logdet, double* sign, void* stream);
+
+/*
  * B := op(A)^-1 B with A n x n triangular (upper != 0: upper), unit != 0
  * for an implicit unit diagonal, trans != 0 for A^T.  B is n x nrhs.
  * Replaces: tmv::TriLDivEq(Tri, MatrixView) (include/tmv/TMV_TriDiv.h:72-75,

@@ -12,6 +12,9 @@
 //   m.setDiv(), m.lud() -> LUDiv<T> with getL/getU/getP/det/logDet/
 //   isSingular/isTrans                             (TMV_Matrix.h:933-975,
 //                                                   TMV_LUD.h:90-161)
+//   m.inverse(), m.det(), m.logDet(&sign)          (TMV_Matrix.h:869-871,
+//                                                   926-928,
+//                                                   src/TMV_LUD.cpp:133-169)
 //   LU_Decompose(Matrix&, Permutation&)             (TMV_LUD.h:84-88)
 // Like TMV, the factorisation is made lazily by the first division, reused
 // while saveDiv() is set and discarded after each division otherwise
@@ -101,6 +104,12 @@ template <> struct Abi<double> {
     static int getrs(int tr, int64_t n, int64_t nr, const double* LU, int64_t lda, const int64_t* p,
                      double* B, int64_t ldb)
     { return tmvx_getrs_f64(tr, n, nr, LU, lda, p, B, ldb, 1, nullptr); }
+    static int getri(int64_t n, const double* LU, int64_t lda, const int64_t* p, double* X,
+                     int64_t ldx)
+    { return tmvx_getri_f64(n, LU, lda, p, X, ldx, nullptr); }
+    static int logdet(int64_t n, const double* LU, int64_t lda, const int64_t* p, double* ld,
+                      double* sg)
+    { return tmvx_logdet_f64(n, LU, lda, p, ld, sg, nullptr); }
 };
 template <> struct Abi<float> {
     static int gemm(int la, int lb, int64_t m, int64_t n, int64_t k, float al, const float* A,
@@ -111,6 +120,12 @@ template <> struct Abi<float> {
     static int getrs(int tr, int64_t n, int64_t nr, const float* LU, int64_t lda, const int64_t* p,
                      float* B, int64_t ldb)
     { return tmvx_getrs_f32(tr, n, nr, LU, lda, p, B, ldb, 1, nullptr); }
+    static int getri(int64_t n, const float* LU, int64_t lda, const int64_t* p, float* X,
+                     int64_t ldx)
+    { return tmvx_getri_f32(n, LU, lda, p, X, ldx, nullptr); }
+    static int logdet(int64_t n, const float* LU, int64_t lda, const int64_t* p, double* ld,
+                      double* sg)
+    { return tmvx_logdet_f32(n, LU, lda, p, ld, sg, nullptr); }
 };
 
 } // namespace detail
@@ -215,6 +230,26 @@ public:
         doneDiv();
         return d;
     }
+    T logDet(T* sign = nullptr) const
+    {
+        if (m_ == 0) { if (sign) *sign = T(1); return T(0); }
+        setDiv();
+        const T ld = div_->logDet(sign);
+        doneDiv();
+        return ld;
+    }
+    // m.inverse() with LU division (TMV_Matrix.h:926-928, LUDiv::
+    // makeInverse): a new matrix of the same storage order.
+    Matrix inverse() const
+    {
+        if (!isSquare()) throw Error("inverse of a non-square matrix needs QR or SV division");
+        Matrix R(m_, n_);
+        if (m_ == 0) return R;
+        setDiv();
+        div_->makeInverse(R.ptr());
+        doneDiv();
+        return R;
+    }
 
 private:
     ptrdiff_t m_, n_;
@@ -269,19 +304,28 @@ public:
             for (ptrdiff_t i = j + 1; i < n_; ++i) M(i, j) = T(0);
         return M;
     }
-    // log|det| and sign (src/TMV_LUD.cpp:132-156).
+    // log|det| and sign (src/TMV_LUD.cpp:145-156), reduced on the device;
+    // the factors of A^T have the same determinant.
     T logDet(T* sign = nullptr) const
     {
-        Matrix<T, ColMajor> M = getLU();
-        T ld = 0, sg = T(P_.det());
-        for (ptrdiff_t i = 0; i < n_; ++i) {
-            const T d = M(i, i);
-            if (d == T(0)) { ld = -std::numeric_limits<T>::infinity(); sg = T(0); break; }
-            ld += std::log(std::fabs(d));
-            if (d < T(0)) sg = -sg;
-        }
-        if (sign) *sign = sg;
-        return ld;
+        double ld = 0.0, sg = 1.0;
+        if (n_ > 0)
+            detail::check(detail::Abi<T>::logdet(n_, lu_.get(), n_, piv_.get(), &ld, &sg),
+                          "LUDiv::logDet");
+        if (sign) *sign = T(sg);
+        return T(ld);
+    }
+    // A^-1 written into n x n storage laid out like A (src/TMV_LUD.cpp:
+    // 158-169).  For a row-major A the factors are those of A^T, whose
+    // column-major inverse is A^-1 in row-major order, as TMV's
+    // LU_Inverse(LUx, P, minv.transpose()) forms it.
+    void makeInverse(T* out) const
+    {
+        if (n_ == 0) return;
+        detail::DevBuf<T> x((size_t)(n_ * n_));
+        detail::check(detail::Abi<T>::getri(n_, lu_.get(), n_, piv_.get(), x.get(), n_),
+                      "LU_Inverse");
+        x.download(out, (size_t)(n_ * n_));
     }
     T det() const
     {

@@ -217,6 +217,57 @@ def det_from_lu(LU, P) -> float:
     return f(LU.shape[0], _ptr(LU), LU.shape[0], _ptr(P))
 
 
+def logdet_from_lu(LU, P) -> tuple[float, float]:
+    """(log|det|, sign) from the factors (LUDiv::logDet)."""
+    f = oracle_lib().fn("logdet_from_lu_f64")
+    f.restype = ctypes.c_double
+    f.argtypes = [_i64, _vp, _i64, _vp, ctypes.c_void_p]
+    LU = _fortran(LU, np.float64)
+    P = np.ascontiguousarray(P, np.int64)
+    sign = ctypes.c_double(0.0)
+    ld = f(LU.shape[0], _ptr(LU), LU.shape[0], _ptr(P), ctypes.addressof(sign))
+    return ld, sign.value
+
+
+def inverse_from_lu(LU, P, dtype=np.float64):
+    """A^-1 from the factors: the solve of A X = I (the matrix TMV's
+    LU_Inverse returns, src/TMV_LUInverse.cpp:235-275)."""
+    n = np.asarray(LU).shape[0]
+    return getrs(LU, P, np.eye(n), dtype=dtype)
+
+
+def ref_inverse(A):
+    """A.inverse() through the real TMV (LU division)."""
+    lib = ref_lib()
+    if lib is None:
+        raise FileNotFoundError(REF_SO)
+    A = _fortran(A, np.float64)
+    n = A.shape[0]
+    X = np.zeros((n, n), order="F")
+    f = lib.fn("inverse_f64")
+    f.argtypes = [_i64, _vp, _i64, _vp, _i64]
+    rc = f(n, _ptr(A), n, _ptr(X), n)
+    if rc == -2:
+        raise SingularError("singular")
+    if rc != 0:
+        raise RuntimeError(f"inverse failed: {rc}")
+    return X
+
+
+def ref_logdet(A) -> tuple[float, float]:
+    """A.logDet(&sign) through the real TMV (LU division)."""
+    lib = ref_lib()
+    if lib is None:
+        raise FileNotFoundError(REF_SO)
+    A = _fortran(A, np.float64)
+    out, sign = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    f = lib.fn("logdet_f64")
+    f.argtypes = [_i64, _vp, _i64, ctypes.c_void_p, ctypes.c_void_p]
+    if f(A.shape[0], _ptr(A), A.shape[0], ctypes.addressof(out), ctypes.addressof(sign)) != 0:
+        raise RuntimeError("logdet failed")
+    return out.value, sign.value
+
+
 def set_threads(n: int) -> None:
     for lib in (oracle_lib(), ref_lib()):
         if lib is not None:

@@ -516,4 +516,27 @@ double orc_det_from_lu_f64(idx n, const double* LU, idx lda, const idx* P)
     return (double)d;
 }
 
+// log|det| and sign from the factors, the pair LUDiv::logDet returns
+// (src/TMV_LUD.cpp:145-156): DiagMatrix::logDet (src/TMV_DiagMatrix.cpp:
+// 59-90) sums log|U_ii| and multiplies the signs, a zero diagonal setting
+// log|det| = log(0) = -inf and sign 0; det(P) flips the sign once per
+// interchange.  Summed in long double.
+double orc_logdet_from_lu_f64(idx n, const double* LU, idx lda, const idx* P, double* sign)
+{
+    long double sum = 0.0L;
+    double s = 1.0;
+    bool zero = false;
+    for (idx i = 0; i < n; ++i) {
+        const double d = LU[i + i * lda];
+        if (d == 0.0) zero = true;
+        else {
+            sum += logl(fabsl((long double)d));
+            if (d < 0.0) s = -s;
+        }
+        if (P[i] != i) s = -s;
+    }
+    *sign = zero ? 0.0 : s;
+    return zero ? -INFINITY : (double)sum;
+}
+
 } // extern "C"

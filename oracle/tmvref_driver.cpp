@@ -114,6 +114,35 @@ int det(ptrdiff_t n, const T* A, ptrdiff_t lda, T* out)
     return 0;
 }
 
+// A^-1 through A.inverse() with LU division (src/TMV_LUD.cpp:158-169,
+// src/TMV_LUInverse.cpp:235-275).
+template <class T>
+int inverse(ptrdiff_t n, const T* A, ptrdiff_t lda, T* X, ptrdiff_t ldx)
+{
+    try {
+        tmv::Matrix<T, tmv::ColMajor> Am(n, n);
+        Am = cview(A, n, n, lda, 0);
+        Am.divideUsing(tmv::LU);
+        tmv::Matrix<T, tmv::ColMajor> Xm = Am.inverse();
+        mview(X, n, n, ldx, 0) = Xm;
+    } catch (tmv::Singular&) { return -2;
+    } catch (...) { return -1; }
+    return 0;
+}
+
+// log|det A| and its sign through A.logDet(&sign) (src/TMV_LUD.cpp:145-156).
+template <class T>
+int logdet(ptrdiff_t n, const T* A, ptrdiff_t lda, T* out, T* sign)
+{
+    try {
+        tmv::Matrix<T, tmv::ColMajor> Am(n, n);
+        Am = cview(A, n, n, lda, 0);
+        Am.divideUsing(tmv::LU);
+        *out = Am.logDet(sign);
+    } catch (...) { return -1; }
+    return 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -153,5 +182,11 @@ int tmvref_gesv_f32(ptrdiff_t n, ptrdiff_t nrhs, const float* A, ptrdiff_t lda,
 
 int tmvref_det_f64(ptrdiff_t n, const double* A, ptrdiff_t lda, double* out)
 { return det<double>(n, A, lda, out); }
+
+int tmvref_inverse_f64(ptrdiff_t n, const double* A, ptrdiff_t lda, double* X, ptrdiff_t ldx)
+{ return inverse<double>(n, A, lda, X, ldx); }
+
+int tmvref_logdet_f64(ptrdiff_t n, const double* A, ptrdiff_t lda, double* out, double* sign)
+{ return logdet<double>(n, A, lda, out, sign); }
 
 } // extern "C"

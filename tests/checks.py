@@ -47,3 +47,31 @@ def backward_error(A, x, b):
 def pivot_gap_ok(gaps, n, eps, scale=64.0):
     """True where the pivot choice is unambiguous (gap above scale*n*eps)."""
     return np.asarray(gaps) > scale * n * eps
+
+
+def inverse_residual(A, X):
+    """||A X - I||_max / (n ||A||_max ||X||_max): below a few eps for an
+    inverse formed by backward-stable solves (float64 product; long double
+    up to n = 200)."""
+    A = np.asarray(A)
+    X = np.asarray(X)
+    n = A.shape[0]
+    if n == 0:
+        return 0.0
+    if n <= 200:
+        R = A.astype(np.longdouble) @ X.astype(np.longdouble) - np.eye(n, dtype=np.longdouble)
+    else:
+        R = A.astype(np.float64) @ X.astype(np.float64) - np.eye(n)
+    den = n * max(float(np.abs(A).max()), 1e-300) * max(float(np.abs(X).max()), 1e-300)
+    return float(np.abs(R).max()) / den
+
+
+def inverse_close(X, Xref, A, eps, factor=64.0):
+    """||X - Xref||_max <= factor n eps kappa ||Xref||_max with kappa =
+    ||A||_1 ||Xref||_1: two inverses from backward-stable LU agree to the
+    conditioning of A."""
+    n = A.shape[0]
+    kappa = np.abs(A).sum(axis=0).max() * np.abs(Xref).sum(axis=0).max()
+    bound = factor * max(n, 1) * eps * kappa * np.abs(Xref).max()
+    err = float(np.abs(np.asarray(X, np.float64) - Xref).max())
+    return err <= bound, err / bound

@@ -194,6 +194,43 @@ static void TestSingular()
     Assert(thrown, "b/m on a singular matrix throws tmv::Singular");
 }
 
+// examples/Division.cpp:75-83: A2.inverse() to the 6 printed digits, in
+// both storage orders (a row-major matrix is factorised as its transpose);
+// m.logDet(&sign) against log|det| of the same matrix; inverse of a
+// singular matrix throws tmv::Singular.
+template <int S>
+static void TestInverse()
+{
+    tmv::Matrix<double, S> A2(4, 4);
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) A2(i, j) = 6. * i - 2 * j * j + 2.;
+    for (int i = 0; i < 4; ++i) A2(i, i) += 5.;
+    A2(0, 0) = 20.;
+    for (int j = 0; j < 4; ++j) A2(0, j) *= 2.;
+    const double g[16] = {0.0210347, -0.0395431, -0.012522, 0.0325132,
+                          -0.016696, 0.0966608, -0.0527241, 0.0316344,
+                          -0.00911687, -0.0643234, 0.139631, -0.0537786,
+                          -0.00153779, -0.0253076, -0.0680141, 0.0608084};
+    tmv::Matrix<double, S> X = A2.inverse();
+    bool ok = true;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            ok = ok && std::fabs(X(i, j) - g[i * 4 + j]) <= 5e-6 * std::fabs(g[i * 4 + j]);
+    Assert(ok, S == tmv::ColMajor ? "A2.inverse() (col-major)" : "A2.inverse() (row-major)");
+    double sg = 0;
+    const double ld = A2.logDet(&sg);
+    Assert(std::fabs(sg * std::exp(ld) - A2.det()) <= 1e-12 * std::fabs(A2.det()) && sg != 0,
+           "logDet(&sign) agrees with det()");
+    tmv::Matrix<double, S> m(20, 20, 0.0);
+    for (int i = 0; i < 20; ++i) for (int j = 0; j < 20; ++j) m(i, j) = 2.0 + 4 * i - 5 * j;
+    for (int i = 0; i < 20; ++i) m(i, 1) = 0;
+    bool thrown = false;
+    try { tmv::Matrix<double, S> mi = m.inverse(); (void)mi; }
+    catch (tmv::Singular&) { thrown = true; }
+    Assert(thrown, "inverse of a singular matrix throws tmv::Singular");
+    Assert(m.det() == 0.0 && m.logDet(&sg) == -std::numeric_limits<double>::infinity() && sg == 0,
+           "singular: det 0, logDet -inf, sign 0");
+}
+
 static void TestMixed()
 {
     const int N = 300;
@@ -224,6 +261,8 @@ int main()
     TestDecomp();
     TestSingular();
     TestMixed();
+    TestInverse<tmv::ColMajor>();
+    TestInverse<tmv::RowMajor>();
     std::printf("%s: %d failure(s)\n", nfail ? "FAILED" : "PASSED", nfail);
     return nfail ? 1 : 0;
 }

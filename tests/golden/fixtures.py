@@ -46,6 +46,17 @@ def division_example() -> tuple[np.ndarray, np.ndarray]:
     return A, b
 
 
+def inverse_example() -> tuple[np.ndarray, np.ndarray]:
+    """examples/Division.cpp:39-62: the division example's A with
+    A(0,0) = 20 (:39), copied to A2 (:48) whose row 0 is then doubled (:61);
+    KNOWN["inverse_example"] holds A2.inverse() (:76-83), 5./A2
+    (:112-119), A2.inverse() * b (:124-126) and b * A2.inverse() (:131-133)."""
+    A, b = division_example()
+    A[0, 0] = 20.0
+    A[0] *= 2.0
+    return A, b
+
+
 def matrix_example() -> tuple[np.ndarray, np.ndarray]:
     """examples/Matrix.cpp: m1 (4x3) as printed just before `m1 * m5`
     (:257-262) and the 3x3 m5 (:87-91, row-major list)."""

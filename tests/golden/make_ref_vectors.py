@@ -54,6 +54,9 @@ def main() -> None:
         v[f"lu_{n}_P"] = P
         b = rng.uniform(SEED + 300 + n, n, -1.0, 1.0)
         v[f"solve_{n}_x"] = oracle.ref_gesv(M, b)
+        # m.inverse() and m.logDet(&sign) with LU division.
+        v[f"inv_{n}"] = oracle.ref_inverse(M)
+        v[f"logdet_{n}"] = np.array(oracle.ref_logdet(M))
     # TMV's own fixtures (tests/TMV_TestMatrixDecomp.cpp): pivots + digest.
     for t in (0, 3, 4, 5, 6):
         LU, P = oracle.getrf(F.decomp_matrix(t), lib=ref)
@@ -75,6 +78,7 @@ def main() -> None:
     v["cfg1_lu_sha256"] = np.array(digest(LU))
     b = rng.uniform(SEED + 2, n, -1.0, 1.0)
     v["cfg1_x"] = oracle.ref_gesv(As, b)
+    v["cfg1_logdet"] = np.array(oracle.ref_logdet(As))
     np.savez_compressed(OUT, **v)
     print(f"wrote {OUT}: {len(v)} arrays, {os.path.getsize(OUT)} bytes")
 

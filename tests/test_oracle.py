@@ -9,7 +9,8 @@ import pytest
 import scipy.linalg as sla
 
 import fixtures as F
-from checks import EPS64, EPS32, gemm_bound_ok, lu_residual, backward_error
+from checks import (EPS64, EPS32, gemm_bound_ok, lu_residual, backward_error,
+                    inverse_residual, inverse_close)
 from tmv_amd import rng
 
 EPS = 10 * EPS64          # TMV's test EPS (tests/TMV_Test.h:24)
@@ -115,6 +116,76 @@ def test_square_div_det_28800(orc):
     assert abs(orc.det_from_lu(LU, P) - F.KNOWN["square_div_det"]) < 1e-9 * 28800
     x = orc.getrs(LU, P, b)
     assert backward_error(m, x, b) < 8 * 4 * EPS64
+
+
+def test_inverse_example_golden(orc):
+    """examples/Division.cpp:75-133: A2.inverse(), 5./A2, A2.inverse() * b
+    and b * A2.inverse() to the 6 printed significant digits."""
+    A2, b = F.inverse_example()
+    K = F.KNOWN["inverse_example"]
+    LU, P = orc.getrf(A2)
+    X = orc.inverse_from_lu(LU, P)
+    np.testing.assert_allclose(X, K["A2inv"], rtol=5e-6)
+    np.testing.assert_allclose(5.0 * X, K["five_over_A2"], rtol=5e-6)
+    np.testing.assert_allclose(X @ b, K["inv_times_b"], rtol=5e-6)
+    np.testing.assert_allclose(b @ X, K["b_times_inv"], rtol=5e-6)
+    # a transposed or unpermuted inverse fails the fixture
+    assert not np.allclose(X.T, K["A2inv"], rtol=5e-6)
+    assert not np.allclose(orc.getrs(LU, np.arange(4), np.eye(4)), K["A2inv"], rtol=5e-6)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 37, 64, 100, 130])
+def test_inverse_matches_tmv_ref_vectors(orc, ref_vectors, n):
+    """The solve of A X = I equals TMV's m.inverse() (LU_Inverse's
+    U^-1 L^-1 P^T) to A's conditioning, and A X = I to a few eps."""
+    M = rng.uniform_matrix(SEED + 200 + n, n, n)
+    LU, P = orc.getrf(M)
+    X = orc.inverse_from_lu(LU, P)
+    ok, r = inverse_close(X, ref_vectors[f"inv_{n}"], M, EPS64)
+    assert ok, r
+    assert inverse_residual(M, X) <= 8 * EPS64
+    assert inverse_residual(M, ref_vectors[f"inv_{n}"]) <= 8 * EPS64
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 37, 64, 100, 130])
+def test_logdet_matches_tmv_ref_vectors(orc, ref_vectors, n):
+    """logDet(&sign) from the factors equals TMV's; independently,
+    LAPACK's slogdet (numpy) agrees."""
+    M = rng.uniform_matrix(SEED + 200 + n, n, n)
+    LU, P = orc.getrf(M)
+    ld, sg = orc.logdet_from_lu(LU, P)
+    ref_ld, ref_sg = ref_vectors[f"logdet_{n}"]
+    assert sg == ref_sg
+    assert abs(ld - ref_ld) <= 4 * n * EPS64 * max(1.0, abs(ref_ld))
+    np_sg, np_ld = np.linalg.slogdet(M)
+    assert sg == np_sg and abs(ld - np_ld) <= 1e-10 * max(1.0, abs(np_ld))
+
+
+def test_logdet_fixtures_and_singular(orc, ref_vectors):
+    """sign * exp(logdet) reproduces TMV's exact determinants
+    (tests/TMV_TestMatrixDet.cpp); the cfg1 matrix A + 512 I (det far beyond
+    double range) matches TMV's logDet; a zero pivot gives (-inf, 0) as
+    DiagMatrix::logDet does (src/TMV_DiagMatrix.cpp:67-69)."""
+    mats = F.det_matrices()
+    for name, (true, k) in F.KNOWN["dets"].items():
+        m = mats[name]
+        LU, P = orc.getrf(m)
+        ld, sg = orc.logdet_from_lu(LU, P)
+        d = 0.0 if sg == 0 else sg * np.exp(ld)
+        norm_m = mats["m11"] if name == "m12" else m
+        assert abs(d - true) <= k * float(np.sum(norm_m ** 2)) * EPS, name
+    n = 512
+    As = rng.uniform_matrix(SEED + 0, n, n) + n * np.eye(n)
+    LU, P = orc.getrf(As)
+    ld, sg = orc.logdet_from_lu(LU, P)
+    assert sg == ref_vectors["cfg1_logdet"][1]
+    assert abs(ld - ref_vectors["cfg1_logdet"][0]) <= 4 * n * EPS64 * ld
+    A = F.decomp_matrix(3, N=20)
+    LU, P = orc.getrf(A)
+    ld, sg = orc.logdet_from_lu(LU, P)
+    assert sg == 0.0 and ld == -np.inf
+    if orc.ref_lib() is not None:
+        assert orc.ref_logdet(A) == (-np.inf, 0.0)
 
 
 def test_division_example_golden(orc):

@@ -13,6 +13,7 @@ a row-major one has stride (ld, 1).
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -46,6 +47,10 @@ _SIGS = {
     "tmvx_getrf_f32": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
     "tmvx_getrs_f64": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _int, _vp]),
     "tmvx_getrs_f32": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _int, _vp]),
+    "tmvx_getri_f64": (_int, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "tmvx_getri_f32": (_int, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "tmvx_logdet_f64": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
+    "tmvx_logdet_f32": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
     "tmvx_trsm_f64": (_int, [_int, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
     "tmvx_trsm_f32": (_int, [_int, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
     "tmvx_laswp_f64": (_int, [_int, _i64, _vp, _i64, _i64, _i64, _vp, _vp]),
@@ -252,6 +257,38 @@ def getrs(LU, piv, B, trans=False, check_singular=True, stream=None):
     _check(fn(int(bool(trans)), n, Bm.shape[1], LU.data_ptr(), lda, piv.data_ptr(),
               Bm.data_ptr(), ldb, int(bool(check_singular)), _stream(stream)), "getrs")
     return B
+
+
+def getri(LU, piv, stream=None):
+    """A^-1 from the factors (TMV LU_Inverse / m.inverse() with LU
+    division); returns a new column-major n x n tensor."""
+    torch = _torch()
+    n = LU.shape[0]
+    lda = _colmajor_ld(LU, "LU")
+    X = colmajor(torch.empty(n, n, dtype=LU.dtype, device=LU.device))
+    fn = getattr(lib(), "tmvx_getri_" + _suffix(LU.dtype))
+    _check(fn(n, LU.data_ptr(), lda, piv.data_ptr(), X.data_ptr(), max(n, 1), _stream(stream)),
+           "getri")
+    return X
+
+
+def logdet(LU, piv, stream=None):
+    """(log|det A|, sign) from the factors (TMV LUDiv::logDet); sign is
+    +1, -1, or 0 for a singular matrix (logdet -inf)."""
+    n = LU.shape[0]
+    lda = _colmajor_ld(LU, "LU")
+    ld, sg = _d(0.0), _d(0.0)
+    fn = getattr(lib(), "tmvx_logdet_" + _suffix(LU.dtype))
+    _check(fn(n, LU.data_ptr(), lda, piv.data_ptr(), ctypes.byref(ld), ctypes.byref(sg),
+              _stream(stream)), "logdet")
+    return ld.value, sg.value
+
+
+def det(LU, piv, stream=None):
+    """det A from the factors, formed as TMV's LUDiv::det does:
+    sign * exp(logdet), 0 when a diagonal is 0."""
+    ld, sg = logdet(LU, piv, stream)
+    return 0.0 if sg == 0.0 else sg * math.exp(ld)
 
 
 def trsm(A, B, upper, trans=False, unit=False, stream=None):
