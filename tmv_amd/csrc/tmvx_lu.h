@@ -16,15 +16,14 @@ template <typename T>
 int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s,
                int lowprec = TMVX_LOWPREC_F32, int tournament = 0);
 
-// Tournament pivoting pieces (tmvx_tslu.hip): selection tree + interchange
-// list for panel [j0, j0+jb), then the factorisation of the panel once its
-// interchanges are applied to its columns.
+// Tournament pivoting (tmvx_tslu.hip): panel [j0, j0+jb) factored in place
+// on its own columns (selection tree, interchange list piv[j0..j0+jb), the
+// interchanges on the panel's columns, L and U); the other columns' share of
+// the interchanges is left to the caller.  ws: tslu_ws_ints(n - j0) ints.
 i64 tslu_ws_ints(i64 m);
 template <typename T>
 int tslu_panel(T* A, i64 lda, i64 n, i64 j0, int jb, i64* piv, i64* first_zero, int* ws,
                hipStream_t s);
-template <typename T>
-int tslu_factor(T* A, i64 lda, i64 n, i64 j0, int jb, i64* first_zero, hipStream_t s);
 
 // C := beta C + alpha A B on FP16 / BF16 MFMA, FP32 operands and result;
 // with bmax_word != nullptr, B is scaled by a power of two fitted to

@@ -607,12 +607,9 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
             if constexpr (sizeof(T) == 4) {
                 if (tslu) {
                     // tournament pivots, their interchanges on the panel's
-                    // own columns, then the panel's L and U
+                    // own columns and the panel's L and U
                     if ((rc = tslu_panel<T>(A, lda, n, j0, jb, piv, wk.first_zero, ts_ws, s)) !=
-                            TMVX_OK ||
-                        (rc = swap_cols(j0, jb, j0, j0 + jb, j0 + jb, j0 + jb, nullptr, s)) !=
-                            TMVX_OK ||
-                        (rc = tslu_factor<T>(A, lda, n, j0, jb, wk.first_zero, s)) != TMVX_OK)
+                        TMVX_OK)
                         break;
                 }
             }

