@@ -104,6 +104,27 @@ int tmvx_getrs_f32(int trans, int64_t n, int64_t nrhs, const float* LU,
                    int check_singular, void* stream);
 
 /*
+ * Complex GEMM: C := alpha * op(A) * op(B) + beta * C with interleaved
+ * complex storage (z: std::complex<double>, c: std::complex<float>; the
+ * pointers address the first real part, leading dimensions count complex
+ * elements, every pointer aligned to a whole complex element).  op(X) is X,
+ * or its conjugate when conj_x != 0 (TMV's conjugate views); layouts as in
+ * tmvx_gemm_f64; C column-major.  beta == 0: C is not read.  Computed as
+ * TMV does, from four real products of the real and imaginary parts with
+ * the conjugations folded into their signs, here on the real MFMA GEMM.
+ * Replaces: tmv::MultMM<add> for complex T (include/tmv/TMV_MatrixArithFunc.h:
+ * 53-57) and its complex block split (src/TMV_MultMM_Block.cpp:334-392).
+ */
+int tmvx_gemm_z(int layout_a, int layout_b, int conj_a, int conj_b, int64_t m, int64_t n,
+                int64_t k, double alpha_re, double alpha_im, const double* A, int64_t lda,
+                const double* B, int64_t ldb, double beta_re, double beta_im, double* C,
+                int64_t ldc, void* stream);
+int tmvx_gemm_c(int layout_a, int layout_b, int conj_a, int conj_b, int64_t m, int64_t n,
+                int64_t k, float alpha_re, float alpha_im, const float* A, int64_t lda,
+                const float* B, int64_t ldb, float beta_re, float beta_im, float* C,
+                int64_t ldc, void* stream);
+
+/*
  * X := A^-1 from the factors of tmvx_getrf_* (X n x n column-major, not
  * aliasing LU).  The same matrix TMV's LU_Inverse returns (U^-1 L^-1 with
  * the columns reverse-permuted by piv); formed here as the solve of

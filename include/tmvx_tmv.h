@@ -12,6 +12,9 @@
 //   m.setDiv(), m.lud() -> LUDiv<T> with getL/getU/getP/det/logDet/
 //   isSingular/isTrans                             (TMV_Matrix.h:933-975,
 //                                                   TMV_LUD.h:90-161)
+//   complex products: Matrix<std::complex<double|float>> with *, MultMM
+//                                                  (src/TMV_MultMM_Block.cpp:
+//                                                   334-392)
 //   m.inverse(), m.det(), m.logDet(&sign)          (TMV_Matrix.h:869-871,
 //                                                   926-928,
 //                                                   src/TMV_LUD.cpp:133-169)
@@ -30,6 +33,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cmath>
+#include <complex>
 #include <cstddef>
 #include <cstdint>
 #include <limits>
@@ -126,6 +130,31 @@ template <> struct Abi<float> {
     static int logdet(int64_t n, const float* LU, int64_t lda, const int64_t* p, double* ld,
                       double* sg)
     { return tmvx_logdet_f32(n, LU, lda, p, ld, sg, nullptr); }
+};
+
+// Complex element types: products only (TMV's complex LU is not on this
+// library's path), through the complex GEMM entry points.
+template <> struct Abi<std::complex<double>> {
+    typedef std::complex<double> Z;
+    static int gemm(int la, int lb, int64_t m, int64_t n, int64_t k, Z al, const Z* A,
+                    int64_t lda, const Z* B, int64_t ldb, Z be, Z* C, int64_t ldc)
+    {
+        return tmvx_gemm_z(la, lb, 0, 0, m, n, k, al.real(), al.imag(),
+                           reinterpret_cast<const double*>(A), lda,
+                           reinterpret_cast<const double*>(B), ldb, be.real(), be.imag(),
+                           reinterpret_cast<double*>(C), ldc, nullptr);
+    }
+};
+template <> struct Abi<std::complex<float>> {
+    typedef std::complex<float> Z;
+    static int gemm(int la, int lb, int64_t m, int64_t n, int64_t k, Z al, const Z* A,
+                    int64_t lda, const Z* B, int64_t ldb, Z be, Z* C, int64_t ldc)
+    {
+        return tmvx_gemm_c(la, lb, 0, 0, m, n, k, al.real(), al.imag(),
+                           reinterpret_cast<const float*>(A), lda,
+                           reinterpret_cast<const float*>(B), ldb, be.real(), be.imag(),
+                           reinterpret_cast<float*>(C), ldc, nullptr);
+    }
 };
 
 } // namespace detail

@@ -116,6 +116,69 @@ def gemm(A, B, C=None, alpha=1.0, add=False, a_rowmajor=False,
     return C
 
 
+def gemm_complex(A, B, C=None, alpha=1.0, add=False, conj_a=False, conj_b=False,
+                 a_rowmajor=False, b_rowmajor=False, dtype=np.complex128):
+    """Complex C (+)= alpha*op(A)*op(B) as TMV computes it: four real products
+    of the real and imaginary parts, conjugations folded into the signs
+    (DoBlockMultMM's complex overloads, src/TMV_MultMM_Block.cpp:370-392),
+    each real product by the real restatement above.  A complex alpha scales
+    each real product into both parts of C (the real-product, complex-C
+    kernel adds x * tile)."""
+    rdt = np.float64 if np.dtype(dtype) == np.complex128 else np.float32
+    A = np.asarray(A)
+    B = np.asarray(B)
+    Ar, Ai = np.ascontiguousarray(A.real), np.ascontiguousarray(A.imag)
+    Br, Bi = np.ascontiguousarray(B.real), np.ascontiguousarray(B.imag)
+    m, n = A.shape[0], B.shape[1]
+    if C is None:
+        Cr = np.zeros((m, n), rdt)
+        Ci = np.zeros((m, n), rdt)
+    else:
+        Cr, Ci = np.asarray(C).real.copy(), np.asarray(C).imag.copy()
+    kw = dict(a_rowmajor=a_rowmajor, b_rowmajor=b_rowmajor, dtype=rdt)
+    x = complex(alpha)
+    if x.imag == 0.0:
+        xr = x.real
+        Cr = gemm(Ar, Br, C=Cr, alpha=xr, add=add, **kw)
+        Cr = gemm(Ai, Bi, C=Cr, alpha=-xr if conj_a == conj_b else xr, add=True, **kw)
+        Ci = gemm(Ar, Bi, C=Ci, alpha=-xr if conj_b else xr, add=add, **kw)
+        Ci = gemm(Ai, Br, C=Ci, alpha=-xr if conj_a else xr, add=True, **kw)
+    else:
+        ix = 1j * x
+        terms = [(Ar, Br, x, add), (Ai, Bi, -x if conj_a == conj_b else x, True),
+                 (Ar, Bi, -ix if conj_b else ix, True), (Ai, Br, -ix if conj_a else ix, True)]
+        for P, Q, y, ad in terms:
+            Cr = gemm(P, Q, C=Cr, alpha=y.real, add=ad, **kw)
+            Ci = gemm(P, Q, C=Ci, alpha=y.imag, add=ad, **kw)
+    out = np.empty((m, n), np.dtype(dtype), order="F")
+    out.real, out.imag = Cr, Ci
+    return out
+
+
+def ref_gemm_complex(A, B, C=None, alpha=1.0, add=False, conj_a=False, conj_b=False,
+                     a_rowmajor=False, b_rowmajor=False):
+    """Complex MultMM<add> through the real TMV (complex<double>)."""
+    lib = ref_lib()
+    if lib is None:
+        raise FileNotFoundError(REF_SO)
+    A = np.ascontiguousarray(A, np.complex128) if a_rowmajor else np.asfortranarray(A, np.complex128)
+    B = np.ascontiguousarray(B, np.complex128) if b_rowmajor else np.asfortranarray(B, np.complex128)
+    m, k = A.shape
+    n = B.shape[1]
+    C = np.zeros((m, n), np.complex128, order="F") if C is None else np.array(C, np.complex128, order="F")
+    lda = A.shape[1] if a_rowmajor else max(1, m)
+    ldb = B.shape[1] if b_rowmajor else max(1, k)
+    x = complex(alpha)
+    f = lib.fn("gemm_z")
+    f.argtypes = [ctypes.c_int] * 4 + [_i64] * 3 + [ctypes.c_double] * 2 + \
+        [_vp, _i64, _vp, _i64, ctypes.c_int, _vp, _i64]
+    rc = f(int(a_rowmajor), int(b_rowmajor), int(conj_a), int(conj_b), m, n, k, x.real, x.imag,
+           _ptr(A), lda, _ptr(B), ldb, int(add), _ptr(C), max(1, m))
+    if rc != 0:
+        raise RuntimeError(f"complex gemm failed: {rc}")
+    return C
+
+
 def getrf(A, dtype=np.float64, gaps=False, lib: _Lib | None = None):
     """In-place PA = LU (TMV LU_Decompose).  Returns (LU, P) with P the
     0-based interchange list; with gaps=True also the per-column relative

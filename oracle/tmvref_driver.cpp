@@ -48,6 +48,27 @@ int gemm(int a_rm, int b_rm, ptrdiff_t m, ptrdiff_t n, ptrdiff_t k, T alpha,
     return 0;
 }
 
+// Complex C (+)= alpha*op(A)*op(B) through tmv::MultMM<add> on
+// std::complex views, op = conjugate() when conj_x (TMV's complex block split,
+// src/TMV_MultMM_Block.cpp:334-392).
+template <class RT>
+int gemm_cplx(int a_rm, int b_rm, int conj_a, int conj_b, ptrdiff_t m, ptrdiff_t n,
+              ptrdiff_t k, RT are, RT aim, const RT* A, ptrdiff_t lda, const RT* B,
+              ptrdiff_t ldb, int add, RT* C, ptrdiff_t ldc)
+{
+    typedef std::complex<RT> CT;
+    try {
+        const tmv::ConstMatrixView<CT> A0 = cview(reinterpret_cast<const CT*>(A), m, k, lda, a_rm);
+        const tmv::ConstMatrixView<CT> B0 = cview(reinterpret_cast<const CT*>(B), k, n, ldb, b_rm);
+        const tmv::ConstMatrixView<CT> Av = conj_a ? A0.conjugate() : A0;
+        const tmv::ConstMatrixView<CT> Bv = conj_b ? B0.conjugate() : B0;
+        tmv::MatrixView<CT> Cv = mview(reinterpret_cast<CT*>(C), m, n, ldc, 0);
+        if (add) tmv::MultMM<true>(CT(are, aim), Av, Bv, Cv);
+        else tmv::MultMM<false>(CT(are, aim), Av, Bv, Cv);
+    } catch (...) { return -1; }
+    return 0;
+}
+
 // In-place PA=LU through tmv::LU_Decompose (src/TMV_LUDecompose.cpp:514-559).
 template <class T>
 int getrf(ptrdiff_t n, T* A, ptrdiff_t lda, ptrdiff_t* P)
@@ -160,6 +181,11 @@ int tmvref_gemm_f32(int a_rm, int b_rm, ptrdiff_t m, ptrdiff_t n, ptrdiff_t k,
                     float alpha, const float* A, ptrdiff_t lda, const float* B,
                     ptrdiff_t ldb, int add, float* C, ptrdiff_t ldc)
 { return gemm<float>(a_rm, b_rm, m, n, k, alpha, A, lda, B, ldb, add, C, ldc); }
+
+int tmvref_gemm_z(int a_rm, int b_rm, int conj_a, int conj_b, ptrdiff_t m, ptrdiff_t n,
+                  ptrdiff_t k, double are, double aim, const double* A, ptrdiff_t lda,
+                  const double* B, ptrdiff_t ldb, int add, double* C, ptrdiff_t ldc)
+{ return gemm_cplx<double>(a_rm, b_rm, conj_a, conj_b, m, n, k, are, aim, A, lda, B, ldb, add, C, ldc); }
 
 int tmvref_getrf_f64(ptrdiff_t n, double* A, ptrdiff_t lda, ptrdiff_t* P)
 { return getrf<double>(n, A, lda, P); }

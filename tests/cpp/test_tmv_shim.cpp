@@ -231,6 +231,29 @@ static void TestInverse()
            "singular: det 0, logDet -inf, sign 0");
 }
 
+// Complex products (TMV splits them into four real ones): integer parts,
+// so the GPU result is exact, in both storage orders of C.
+template <int SC>
+static void TestComplexProduct()
+{
+    typedef std::complex<double> Z;
+    tmv::Matrix<Z> a(3, 2), b(2, 2);
+    a(0, 0) = Z(1, 2); a(0, 1) = Z(0, -1); a(1, 0) = Z(3, 0); a(1, 1) = Z(-2, 4);
+    a(2, 0) = Z(5, -5); a(2, 1) = Z(1, 1);
+    b(0, 0) = Z(2, 1); b(0, 1) = Z(-1, 0); b(1, 0) = Z(0, 3); b(1, 1) = Z(4, -2);
+    tmv::Matrix<Z, SC> c(3, 2, Z(1, -1));
+    tmv::MultMM<true>(Z(0, 1), a, b, c);                // c += i a b
+    bool ok = true;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 2; ++j) {
+            Z s(0, 0);
+            for (int l = 0; l < 2; ++l) s += a(i, l) * b(l, j);
+            ok = ok && c(i, j) == Z(1, -1) + Z(0, 1) * s;
+        }
+    Assert(ok, SC == tmv::ColMajor ? "complex MultMM<true> (col-major C)"
+                                   : "complex MultMM<true> (row-major C)");
+}
+
 static void TestMixed()
 {
     const int N = 300;
@@ -263,6 +286,8 @@ int main()
     TestMixed();
     TestInverse<tmv::ColMajor>();
     TestInverse<tmv::RowMajor>();
+    TestComplexProduct<tmv::ColMajor>();
+    TestComplexProduct<tmv::RowMajor>();
     std::printf("%s: %d failure(s)\n", nfail ? "FAILED" : "PASSED", nfail);
     return nfail ? 1 : 0;
 }

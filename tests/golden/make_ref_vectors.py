@@ -32,6 +32,16 @@ def digest(a: np.ndarray) -> str:
     return hashlib.sha256(np.asfortranarray(a).tobytes(order="F")).hexdigest()
 
 
+def digest_c(z: np.ndarray) -> str:
+    """SHA-256 of a complex matrix's column-major interleaved bytes."""
+    return hashlib.sha256(np.asfortranarray(z, np.complex128).tobytes(order="F")).hexdigest()
+
+
+# (tag, conj_a, conj_b, alpha, add) for the complex GEMM vectors
+ZGEMM_VARIANTS = [("n_n_real", 0, 0, 1.0, False), ("c_n_add", 1, 0, -0.5, True),
+                  ("n_c_cplx", 0, 1, 0.3 + 0.7j, False), ("c_c_add", 1, 1, 2.0, True)]
+
+
 def main() -> None:
     ref = oracle.ref_lib()
     if ref is None:
@@ -46,6 +56,17 @@ def main() -> None:
             v[f"gemm_64x48x80_{ar}{br}"] = oracle.gemm(A, B, a_rowmajor=ar, b_rowmajor=br, lib=ref)
     C0 = rng.uniform_matrix(SEED + 103, 64, 48)
     v["gemm_64x48x80_add"] = oracle.gemm(A, B, C=C0, alpha=-0.5, add=True, lib=ref)
+    # Complex GEMM (TMV's four-real-product block path: aligned shape) as
+    # digests, and one ragged (small-kernel path) output.
+    A = rng.uniform_matrix(SEED + 111, 128, 160) + 1j * rng.uniform_matrix(SEED + 112, 128, 160)
+    B = rng.uniform_matrix(SEED + 113, 160, 96) + 1j * rng.uniform_matrix(SEED + 114, 160, 96)
+    C0 = rng.uniform_matrix(SEED + 115, 128, 96) + 1j * rng.uniform_matrix(SEED + 116, 128, 96)
+    for tag, ca, cb, alpha, add in ZGEMM_VARIANTS:
+        Z = oracle.ref_gemm_complex(A, B, C=C0, alpha=alpha, add=add, conj_a=ca, conj_b=cb)
+        v[f"zgemm_128x96x160_{tag}_sha256"] = np.array(digest_c(Z))
+    A = rng.uniform_matrix(SEED + 121, 37, 41) + 1j * rng.uniform_matrix(SEED + 122, 37, 41)
+    B = rng.uniform_matrix(SEED + 123, 41, 29) + 1j * rng.uniform_matrix(SEED + 124, 41, 29)
+    v["zgemm_37x29x41_conjA"] = oracle.ref_gemm_complex(A, B, alpha=0.3 + 0.7j, conj_a=True)
     # LU on uniform [-1,1] inputs without a diagonal shift (pivoting active).
     for n in (1, 2, 3, 5, 37, 64, 100, 130):
         M = rng.uniform_matrix(SEED + 200 + n, n, n)

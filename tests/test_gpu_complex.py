@@ -1,0 +1,119 @@
+"""Complex GEMM (tmvx_gemm_z / tmvx_gemm_c) through the C ABI vs the oracle's
+restatement of TMV's complex product (four real products).
+
+Bar: exact on integer-valued parts (every product and sum exact); elsewhere
+|C - C_ref| <= 4 k u (|A|_1 |B|_1) |alpha| + 2 u |beta||C0| elementwise,
+|.|_1 = |re| + |im| and u the unit roundoff of the real type, in every
+layout pair and conjugation; TMV's own output (bit-exact with the oracle on
+aligned shapes) to the same bound; torch's complex product at 4096."""
+import numpy as np
+import pytest
+import torch
+
+import tmv_amd
+from checks import EPS32, EPS64
+from tmv_amd import rng
+
+pytestmark = pytest.mark.gpu
+SEED = 0x7A15EED
+
+
+def cplx(seed, m, n):
+    return rng.uniform_matrix(seed, m, n) + 1j * rng.uniform_matrix(seed + 1, m, n)
+
+
+def to_dev(z, dtype, rowmajor=False):
+    t = torch.from_numpy(np.ascontiguousarray(z)).to(device="cuda", dtype=dtype)
+    return t if rowmajor else tmv_amd.colmajor(t)
+
+
+def abs1(z):
+    return np.abs(z.real) + np.abs(z.imag)
+
+
+def check(Z, ref, A, B, alpha, beta, C0, eps, k):
+    bound = 4 * max(k, 1) * eps * abs(alpha) * (abs1(A) @ abs1(B)) + 1e-300
+    if C0 is not None and beta != 0:
+        bound = bound + 2 * eps * abs(beta) * np.abs(C0)
+    err = np.abs(Z - ref)
+    return float((err / bound).max())
+
+
+@pytest.mark.parametrize("la,lb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("ca,cb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_zgemm_vs_oracle(orc, dev, la, lb, ca, cb):
+    m, n, k = 133, 70, 97
+    print(f"zgemm layouts {la}{lb} conj {ca}{cb}", flush=True)
+    A, B, C0 = cplx(SEED + 1, m, k), cplx(SEED + 3, k, n), cplx(SEED + 5, m, n)
+    alpha, beta = 0.75 - 0.25j, -0.5 + 1.0j
+    dA, dB = to_dev(A, torch.complex128, la), to_dev(B, torch.complex128, lb)
+    dC = to_dev(C0, torch.complex128)
+    tmv_amd.gemm_complex(dA, dB, dC, alpha=alpha, beta=beta, conj_a=ca, conj_b=cb)
+    Z = dC.cpu().numpy()
+    # oracle: C = alpha op(A) op(B), then + beta C0
+    ref = orc.gemm_complex(A, B, alpha=alpha, conj_a=ca, conj_b=cb) + beta * C0
+    assert check(Z, ref, A, B, alpha, beta, C0, EPS64, k) <= 1.0
+
+
+@pytest.mark.parametrize("ca,cb", [(0, 0), (1, 1)])
+def test_cgemm_vs_oracle(orc, dev, ca, cb):
+    m, n, k = 100, 130, 257
+    A, B = cplx(SEED + 11, m, k), cplx(SEED + 13, k, n)
+    A32, B32 = A.astype(np.complex64), B.astype(np.complex64)
+    dA, dB = to_dev(A32, torch.complex64), to_dev(B32, torch.complex64)
+    Z = tmv_amd.gemm_complex(dA, dB, alpha=1.0 - 2.0j, conj_a=ca, conj_b=cb).cpu().numpy()
+    ref = orc.gemm_complex(A32, B32, alpha=1.0 - 2.0j, conj_a=ca, conj_b=cb,
+                           dtype=np.complex64).astype(np.complex128)
+    assert check(Z.astype(np.complex128), ref, A32, B32, 1.0 - 2.0j, 0, None, EPS32, k) <= 1.0
+
+
+@pytest.mark.parametrize("ca,cb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_zgemm_integer_exact(dev, ca, cb):
+    g = np.random.default_rng(11 + 2 * ca + cb)
+    m, n, k = 65, 47, 129
+    A = g.integers(-9, 10, (m, k)) + 1j * g.integers(-9, 10, (m, k))
+    B = g.integers(-9, 10, (k, n)) + 1j * g.integers(-9, 10, (k, n))
+    C0 = g.integers(-9, 10, (m, n)) + 1j * g.integers(-9, 10, (m, n))
+    dC = to_dev(C0, torch.complex128)
+    tmv_amd.gemm_complex(to_dev(A, torch.complex128), to_dev(B, torch.complex128, True), dC,
+                         alpha=2.0 - 3.0j, beta=1.0 + 1.0j, conj_a=ca, conj_b=cb)
+    Aop = A.conj() if ca else A
+    Bop = B.conj() if cb else B
+    assert np.array_equal(dC.cpu().numpy(), (2.0 - 3.0j) * (Aop @ Bop) + (1.0 + 1.0j) * C0)
+
+
+def test_zgemm_matches_tmv_output(orc, dev, ref_vectors):
+    """TMV's ragged-shape output (its small complex kernels) within the bound."""
+    A = rng.uniform_matrix(SEED + 121, 37, 41) + 1j * rng.uniform_matrix(SEED + 122, 37, 41)
+    B = rng.uniform_matrix(SEED + 123, 41, 29) + 1j * rng.uniform_matrix(SEED + 124, 41, 29)
+    Z = tmv_amd.gemm_complex(to_dev(A, torch.complex128), to_dev(B, torch.complex128),
+                             alpha=0.3 + 0.7j, conj_a=True).cpu().numpy()
+    assert check(Z, ref_vectors["zgemm_37x29x41_conjA"], A, B, 0.3 + 0.7j, 0, None, EPS64,
+                 41) <= 1.0
+
+
+def test_zgemm_edge_sizes(dev):
+    # k = 0: C = beta C; m = 0 / n = 0: nothing to do
+    C0 = cplx(SEED + 31, 5, 4)
+    dC = to_dev(C0, torch.complex128)
+    dA = torch.zeros(5, 0, dtype=torch.complex128, device="cuda")
+    dB = torch.zeros(0, 4, dtype=torch.complex128, device="cuda")
+    tmv_amd.gemm_complex(dA, dB, dC, alpha=1.0, beta=2.0 - 1.0j)
+    np.testing.assert_allclose(dC.cpu().numpy(), (2.0 - 1.0j) * C0, rtol=1e-15)
+    e = torch.zeros(0, 3, dtype=torch.complex128, device="cuda")
+    assert tmv_amd.gemm_complex(e, to_dev(cplx(1, 3, 2), torch.complex128)).shape == (0, 2)
+    A, B = cplx(SEED + 33, 1, 1), cplx(SEED + 35, 1, 1)
+    Z = tmv_amd.gemm_complex(to_dev(A, torch.complex128), to_dev(B, torch.complex128))
+    np.testing.assert_allclose(Z.cpu().numpy(), A @ B, rtol=1e-15)
+
+
+def test_zgemm_4096_vs_torch(dev):
+    """4096^3 complex128 against torch's own complex product (rocBLAS)."""
+    n = 4096
+    g = torch.Generator(device="cuda").manual_seed(5)
+    A = torch.randn(n, n, dtype=torch.complex128, device="cuda", generator=g)
+    B = torch.randn(n, n, dtype=torch.complex128, device="cuda", generator=g)
+    Z = tmv_amd.gemm_complex(tmv_amd.colmajor(A), tmv_amd.colmajor(B))
+    R = A @ B
+    mag = (A.real.abs() + A.imag.abs()) @ (B.real.abs() + B.imag.abs())
+    assert ((Z - R).abs() <= 8 * n * EPS64 * mag).all()

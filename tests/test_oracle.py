@@ -118,6 +118,58 @@ def test_square_div_det_28800(orc):
     assert backward_error(m, x, b) < 8 * 4 * EPS64
 
 
+# ------------------------------------------------------------- complex ----
+
+ZGEMM_VARIANTS = [("n_n_real", 0, 0, 1.0, False), ("c_n_add", 1, 0, -0.5, True),
+                  ("n_c_cplx", 0, 1, 0.3 + 0.7j, False), ("c_c_add", 1, 1, 2.0, True)]
+
+
+def _zdigest(z):
+    return hashlib.sha256(np.asfortranarray(z, np.complex128).tobytes(order="F")).hexdigest()
+
+
+def test_complex_gemm_bit_exact_with_tmv_block_path(orc, ref_vectors):
+    """On an aligned shape TMV takes its block path, which splits a complex
+    product into four real ones (src/TMV_MultMM_Block.cpp:370-392): the
+    restatement (four real oracle GEMMs, same signs and order) reproduces
+    TMV's bytes for every conjugation and alpha kind."""
+    A = rng.uniform_matrix(SEED + 111, 128, 160) + 1j * rng.uniform_matrix(SEED + 112, 128, 160)
+    B = rng.uniform_matrix(SEED + 113, 160, 96) + 1j * rng.uniform_matrix(SEED + 114, 160, 96)
+    C0 = rng.uniform_matrix(SEED + 115, 128, 96) + 1j * rng.uniform_matrix(SEED + 116, 128, 96)
+    for tag, ca, cb, alpha, add in ZGEMM_VARIANTS:
+        Z = orc.gemm_complex(A, B, C=C0, alpha=alpha, add=add, conj_a=ca, conj_b=cb)
+        assert _zdigest(Z) == str(ref_vectors[f"zgemm_128x96x160_{tag}_sha256"]), tag
+
+
+def test_complex_gemm_ragged_close_to_tmv(orc, ref_vectors):
+    """Ragged shapes go through TMV's small complex kernels instead (complex
+    multiply-adds): the four-product form agrees to rounding."""
+    A = rng.uniform_matrix(SEED + 121, 37, 41) + 1j * rng.uniform_matrix(SEED + 122, 37, 41)
+    B = rng.uniform_matrix(SEED + 123, 41, 29) + 1j * rng.uniform_matrix(SEED + 124, 41, 29)
+    Z = orc.gemm_complex(A, B, alpha=0.3 + 0.7j, conj_a=True)
+    ref = ref_vectors["zgemm_37x29x41_conjA"]
+    mag = np.abs(0.3 + 0.7j) * ((np.abs(A.real) + np.abs(A.imag)) @ (np.abs(B.real) + np.abs(B.imag)))
+    assert np.all(np.abs(Z - ref) <= 4 * 41 * EPS64 * mag)
+
+
+@pytest.mark.parametrize("ca,cb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_complex_gemm_integer_exact(orc, ca, cb):
+    """Integer parts: every product and sum is exact, so the result equals
+    numpy's complex product exactly; a dropped conjugation or a swapped
+    sign fails."""
+    g = np.random.default_rng(7 + 2 * ca + cb)
+    A = g.integers(-9, 10, (23, 31)) + 1j * g.integers(-9, 10, (23, 31))
+    B = g.integers(-9, 10, (31, 17)) + 1j * g.integers(-9, 10, (31, 17))
+    C0 = g.integers(-9, 10, (23, 17)) + 1j * g.integers(-9, 10, (23, 17))
+    Aop = A.conj() if ca else A
+    Bop = B.conj() if cb else B
+    for alpha, add in ((1.0, False), (2.0 - 3.0j, True)):
+        Z = orc.gemm_complex(A, B, C=C0, alpha=alpha, add=add, conj_a=ca, conj_b=cb)
+        assert np.array_equal(Z, alpha * (Aop @ Bop) + (C0 if add else 0))
+    Zw = orc.gemm_complex(A, B, conj_a=not ca, conj_b=cb)
+    assert not np.array_equal(Zw, Aop @ Bop)
+
+
 def test_inverse_example_golden(orc):
     """examples/Division.cpp:75-133: A2.inverse(), 5./A2, A2.inverse() * b
     and b * A2.inverse() to the 6 printed significant digits."""

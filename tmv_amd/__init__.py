@@ -47,6 +47,8 @@ _SIGS = {
     "tmvx_getrf_f32": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
     "tmvx_getrs_f64": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _int, _vp]),
     "tmvx_getrs_f32": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _int, _vp]),
+    "tmvx_gemm_z": (_int, [_int, _int, _int, _int, _i64, _i64, _i64, _d, _d, _vp, _i64, _vp, _i64, _d, _d, _vp, _i64, _vp]),
+    "tmvx_gemm_c": (_int, [_int, _int, _int, _int, _i64, _i64, _i64, _f, _f, _vp, _i64, _vp, _i64, _f, _f, _vp, _i64, _vp]),
     "tmvx_getri_f64": (_int, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tmvx_getri_f32": (_int, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tmvx_logdet_f64": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
@@ -211,6 +213,34 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, stream=None):
     fn = getattr(lib(), "tmvx_gemm_" + suf)
     _check(fn(la, lb, m, n, k, alpha, A.data_ptr(), lda, B.data_ptr(), ldb, beta,
               C.data_ptr(), ldc, _stream(stream)), "gemm")
+    return C
+
+
+def gemm_complex(A, B, C=None, alpha=1.0, beta=0.0, conj_a=False, conj_b=False, stream=None):
+    """C := alpha*op(A)*op(B) + beta*C for complex128 / complex64 tensors
+    (TMV MultMM<add> on complex matrices; op = conjugate when conj_x).  A
+    and B may be column- or row-major; C (allocated column-major when None)
+    must be column-major.  Returns C."""
+    torch = _torch()
+    m, k = A.shape
+    k2, n = B.shape
+    if k != k2:
+        raise ValueError("inner dimensions differ")
+    if A.dtype not in (torch.complex128, torch.complex64) or B.dtype != A.dtype:
+        raise TypeError("A and B must share a complex dtype")
+    la, lda = _layout(A, "A")
+    lb, ldb = _layout(B, "B")
+    if C is None:
+        C = empty_colmajor(m, n, A.dtype, A.device)
+        beta = 0.0
+    if C.dtype != A.dtype:
+        raise TypeError("C must have A's dtype")
+    ldc = _colmajor_ld(C, "C")
+    alpha, beta = complex(alpha), complex(beta)
+    fn = lib().tmvx_gemm_z if A.dtype == torch.complex128 else lib().tmvx_gemm_c
+    _check(fn(la, lb, int(bool(conj_a)), int(bool(conj_b)), m, n, k, alpha.real, alpha.imag,
+              A.data_ptr(), lda, B.data_ptr(), ldb, beta.real, beta.imag, C.data_ptr(), ldc,
+              _stream(stream)), "gemm_complex")
     return C
 
 
