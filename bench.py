@@ -302,6 +302,7 @@ def main():
     ap.add_argument("--fp64-reps", type=int, default=2)
     ap.add_argument("--gemm-n", type=int, default=8192)
     ap.add_argument("--gemm-reps", type=int, default=5)
+    ap.add_argument("--zgemm-n", type=int, default=8192, help="complex GEMM size (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-lowp", dest="lowp", action="store_false")
@@ -417,6 +418,34 @@ def main():
     del GA, GB, GC
     torch.cuda.empty_cache()
 
+    # ---- complex GEMM (TMV's four-real-product split), complex128 CCC ------
+    zn = a.zgemm_n
+    zgemm = None
+    if zn > 0:
+        ZA = torch.empty(2 * zn * zn, dtype=torch.float64, device="cuda")
+        ZB = torch.empty(2 * zn * zn, dtype=torch.float64, device="cuda")
+        tmv_amd.fill_uniform(ZA, SEED + 22)
+        tmv_amd.fill_uniform(ZB, SEED + 23)
+        ZAv = torch.view_as_complex(ZA.view(zn, zn, 2)).t()
+        ZBv = torch.view_as_complex(ZB.view(zn, zn, 2)).t()
+        ZC = tmv_amd.empty_colmajor(zn, zn, torch.complex128)
+        tmv_amd.gemm_complex(ZAv, ZBv, ZC)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(a.gemm_reps):
+            tmv_amd.gemm_complex(ZAv, ZBv, ZC)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / a.gemm_reps
+        tf = 8.0 * zn ** 3 / ms / 1e9
+        zgemm = {"n": zn, "ms": round(ms, 3), "tflops": round(tf, 2),
+                 "frac_peak": round(tf / PEAK_FP64_TFLOPS, 4),
+                 "flops": "8 n^3 real (4 real products)"}
+        del ZA, ZB, ZC, ZAv, ZBv
+        torch.cuda.empty_cache()
+
     batched = adjoint = None
     if not a.no_batched:
         batched = {"f64": batched_workload(torch, tmv_amd, torch.float64),
@@ -459,6 +488,7 @@ def main():
         "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
         "mixed_kappa1e2": lowp or None,
         "dgemm_8192": gemm,
+        "zgemm": zgemm,
         "batched_20000": batched,
         "adjoint_512x256": adjoint,
         "roofline": {"bound": "mfma", "achieved": ccc["tflops"], "peak": PEAK_FP64_TFLOPS,
