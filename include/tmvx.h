@@ -125,6 +125,35 @@ int tmvx_gemm_c(int layout_a, int layout_b, int conj_a, int conj_b, int64_t m, i
                 int64_t ldc, void* stream);
 
 /*
+ * Complex LU with partial pivoting, in place, interleaved complex storage
+ * (z: std::complex<double>, c: std::complex<float>; lda in complex
+ * elements).  The pivot is the candidate of largest modulus, first row on
+ * ties; a column whose largest candidate underflows is zeroed from the
+ * diagonal down with piv[j] = j, and *info (host) gets the 1-based first
+ * zero pivot, or 0.  Synchronises.  The complex path is functional, not
+ * tuned: one workgroup per panel, trailing updates on the complex GEMM.
+ * Replaces: tmv::LU_Decompose for complex T (src/TMV_LUDecompose.cpp:217-407,
+ * 443-462).
+ */
+int tmvx_getrf_z(int64_t n, double* A, int64_t lda, int64_t* piv, int64_t* info,
+                 void* stream);
+int tmvx_getrf_c(int64_t n, float* A, int64_t lda, int64_t* piv, int64_t* info,
+                 void* stream);
+
+/*
+ * Solve with the factors of tmvx_getrf_z / _c: trans == 0 solves A X = B,
+ * trans != 0 solves A^T X = B (transpose, not conjugate).  B is n x nrhs
+ * column-major (complex elements), overwritten with X.  A zero U diagonal
+ * met by a non-zero entry reports TMVX_ERR_SINGULAR.  Synchronises.
+ * Replaces: tmv::LU_LDivEq / LU_RDivEq for complex T (src/TMV_LUDiv.cpp:
+ * 185-226, 369-410).
+ */
+int tmvx_getrs_z(int trans, int64_t n, int64_t nrhs, const double* LU, int64_t lda,
+                 const int64_t* piv, double* B, int64_t ldb, void* stream);
+int tmvx_getrs_c(int trans, int64_t n, int64_t nrhs, const float* LU, int64_t lda,
+                 const int64_t* piv, float* B, int64_t ldb, void* stream);
+
+/*
  * X := A^-1 from the factors of tmvx_getrf_* (X n x n column-major, not
  * aliasing LU).  The same matrix TMV's LU_Inverse returns (U^-1 L^-1 with
  * the columns reverse-permuted by piv); formed here as the solve of

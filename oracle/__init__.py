@@ -208,6 +208,46 @@ class SingularError(RuntimeError):
     """A zero diagonal met a non-zero right-hand side (TMV's Singular)."""
 
 
+def getrf_complex(A, gaps=False, lib: _Lib | None = None):
+    """Complex (complex128) PA = LU: the oracle's restatement, or the real
+    TMV with lib=ref_lib().  Returns (LU, P[, gaps])."""
+    lib = lib or oracle_lib()
+    A = np.array(A, np.complex128, order="F")
+    n = A.shape[0]
+    P = np.zeros(n, np.int64)
+    g = np.zeros(n, np.float64)
+    if lib is oracle_lib():
+        f = lib.fn("getrf_z")
+        f.argtypes = [_i64, _i64, _vp, _i64, _vp, _vp]
+        rc = f(n, n, _ptr(A), max(1, n), _ptr(P), _ptr(g) if gaps else None)
+    else:
+        f = lib.fn("getrf_z")
+        f.argtypes = [_i64, _vp, _i64, _vp]
+        rc = f(n, _ptr(A), max(1, n), _ptr(P))
+    if rc != 0:
+        raise RuntimeError(f"complex getrf failed: {rc}")
+    return (A, P, g) if gaps else (A, P)
+
+
+def getrs_complex(LU, P, B, trans=False):
+    """Complex solve with the factors (LU_LDivEq; trans: LU_RDivEq)."""
+    lib = oracle_lib()
+    LU = np.asfortranarray(LU, np.complex128)
+    n = LU.shape[0]
+    B = np.asarray(B, np.complex128)
+    vec = B.ndim == 1
+    X = np.array(B.reshape(n, -1), np.complex128, order="F")
+    P = np.ascontiguousarray(P, np.int64)
+    f = lib.fn("getrs_z")
+    f.argtypes = [ctypes.c_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64]
+    rc = f(int(trans), n, X.shape[1], _ptr(LU), max(1, n), _ptr(P), _ptr(X), max(1, n))
+    if rc == -2:
+        raise SingularError("singular upper triangle")
+    if rc != 0:
+        raise RuntimeError(f"complex getrs failed: {rc}")
+    return X[:, 0] if vec else X
+
+
 def getrs(LU, P, B, trans=False, dtype=np.float64, lib: _Lib | None = None):
     """Solve with the factors (TMV LU_LDivEq; trans -> LU_RDivEq)."""
     lib = lib or oracle_lib()

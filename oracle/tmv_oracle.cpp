@@ -20,6 +20,7 @@
 //   orc_getrs_*  golden solve examples/Division.cpp:22-73, backward error,
 //                and oracle/_ref.
 #include <cmath>
+#include <complex>
 #include <cstddef>
 #include <cstring>
 #include <limits>
@@ -32,6 +33,12 @@
 namespace {
 
 typedef ptrdiff_t idx;
+
+// Real type of an element type and TMV_ABS (std::abs: the modulus for
+// complex elements, include/tmv/TMV_Base.h).
+template <class T> struct Real { typedef T type; };
+template <class R> struct Real<std::complex<R>> { typedef R type; };
+template <class T> typename Real<T>::type oabs(T x) { return std::abs(x); }
 
 // A strided matrix view: element (i,j) at p[i*si + j*sj].
 template <class T> struct View {
@@ -185,20 +192,21 @@ void multmm(bool add, T alpha, const View<T>& A, const View<T>& B,
 // ---------------------------------------------------------------------------
 
 // FindMaxAbsElement (src/TMV_Vector.cpp:473-504): first index wins ties.
-template <class T> T max_abs(const T* v, idx n, idx step, idx* imax)
+template <class T> typename Real<T>::type max_abs(const T* v, idx n, idx step, idx* imax)
 {
-    T best = T(-1);
+    typedef typename Real<T>::type RT;
+    RT best = RT(-1);
     *imax = 0;
     for (idx i = 0; i < n; ++i) {
-        T a = std::fabs(v[i * step]);
+        RT a = oabs(v[i * step]);
         if (a > best) { best = a; *imax = i; }
     }
-    return n > 0 ? best : T(0);
+    return n > 0 ? best : RT(0);
 }
 
 // TMV_Underflow (include/tmv/TMV_Base.h:692-697): |x| < numeric min.
-template <class T> bool underflow(T x)
-{ return std::fabs(x) < std::numeric_limits<T>::min(); }
+template <class RT> bool underflow(RT x)
+{ return std::fabs(x) < std::numeric_limits<RT>::min(); }
 
 // swapRows over a column range (src/TMV_Matrix.cpp:865-930 apply the
 // interchange list in order; block width does not change the result).
@@ -218,14 +226,16 @@ template <class T> void permute_rows(const View<T>& A, const idx* P, idx i1, idx
 // lets tests assert pivot equality only on tie-free inputs.
 struct GapLog { double* out; idx n; };
 
-template <class T> void log_gap(GapLog* g, idx col, const T* v, idx n, idx step, T best)
+template <class T>
+void log_gap(GapLog* g, idx col, const T* v, idx n, idx step, typename Real<T>::type best)
 {
+    typedef typename Real<T>::type RT;
     if (!g || !g->out || col < 0 || col >= g->n) return;
-    if (!(best > T(0))) { g->out[col] = 0.0; return; }
-    T second = T(0);
+    if (!(best > RT(0))) { g->out[col] = 0.0; return; }
+    RT second = RT(0);
     bool seen = false;
     for (idx i = 0; i < n; ++i) {
-        T a = std::fabs(v[i * step]);
+        RT a = oabs(v[i * step]);
         if (a == best && !seen) { seen = true; continue; }
         if (a > second) second = a;
     }
@@ -329,6 +339,7 @@ template <class T> int trsm_upper(const View<T>& U, bool unit, const View<T>& B)
 // TMV_Base.h:719-720; VectorView /= x at TMV_VectorArith.h:105-107).
 template <class T> void recursive_lu(const View<T>& A, idx* P, GapLog* g, idx col0)
 {
+    typedef typename Real<T>::type RT;
     const idx N = A.cols, M = A.rows, R = std::min(N, M);
     if (R > 2) {
         idx N1 = R / 2;
@@ -346,25 +357,25 @@ template <class T> void recursive_lu(const View<T>& A, idx* P, GapLog* g, idx co
         T* a0 = &A(0, 0);
         T* a1 = &A(0, 1);
         idx ip0, ip1;
-        T piv = max_abs(a0, M, A.si, &ip0);
+        RT piv = max_abs(a0, M, A.si, &ip0);
         log_gap(g, col0, a0, M, A.si, piv);
         if (underflow(piv)) {
-            ip0 = 0; piv = T(0);
+            ip0 = 0; piv = RT(0);
             for (idx i = 0; i < M; ++i) a0[i * A.si] = T(0);
         }
-        if (piv != T(0)) {
+        if (piv != RT(0)) {
             if (ip0 != 0) {
                 std::swap(a0[ip0 * A.si], a0[0]);
                 std::swap(a1[ip0 * A.si], a1[0]);
             }
             const T inv = T(1) / a0[0];
             const T a01 = a1[0];
-            piv = T(0);
+            piv = RT(0);
             ip1 = 1;
             for (idx i = 1; i < M; ++i) {
                 a0[i * A.si] *= inv;
                 a1[i * A.si] -= a0[i * A.si] * a01;
-                T ab = std::fabs(a1[i * A.si]);
+                RT ab = oabs(a1[i * A.si]);
                 if (ab > piv) { piv = ab; ip1 = i; }
             }
             log_gap(g, col0 + 1, a1 + A.si, M - 1, A.si, piv);
@@ -374,10 +385,10 @@ template <class T> void recursive_lu(const View<T>& A, idx* P, GapLog* g, idx co
             ++ip1;
         }
         if (underflow(piv)) {
-            piv = T(0); ip1 = 1;
+            piv = RT(0); ip1 = 1;
             for (idx i = 1; i < M; ++i) a1[i * A.si] = T(0);
         }
-        if (piv != T(0) && M > 2) {
+        if (piv != RT(0) && M > 2) {
             if (ip1 != 1) {
                 std::swap(a1[ip1 * A.si], a1[1 * A.si]);
                 std::swap(a0[ip1 * A.si], a0[1 * A.si]);
@@ -396,13 +407,13 @@ template <class T> void recursive_lu(const View<T>& A, idx* P, GapLog* g, idx co
     } else if (R == 1) {
         T* a0 = &A(0, 0);
         idx ip;
-        T piv = max_abs(a0, M, A.si, &ip);
+        RT piv = max_abs(a0, M, A.si, &ip);
         log_gap(g, col0, a0, M, A.si, piv);
         if (underflow(piv)) {
-            piv = T(0); ip = 0;
+            piv = RT(0); ip = 0;
             for (idx i = 0; i < M; ++i) a0[i * A.si] = T(0);
         }
-        if (piv != T(0)) {
+        if (piv != RT(0)) {
             if (ip != 0) std::swap(a0[ip * A.si], a0[0]);
             const T inv = T(1) / a0[0];
             for (idx i = 1; i < M; ++i) a0[i * A.si] *= inv;
@@ -502,6 +513,22 @@ ORC_GEMM(f80, long double)
 ORC_LU(f64, double)
 ORC_LU(f32, float)
 ORC_LU(f80, long double)
+
+// Complex LU and solve (std::complex<double>, interleaved storage): the same
+// recursion with TMV_ABS = the modulus for the pivot search.  Its GEMM steps
+// multiply complex elements directly, where TMV's block path splits them into
+// four real products (src/TMV_MultMM_Block.cpp:370-392): pivots agree, the
+// factors to rounding.
+int orc_getrf_z(idx m, idx n, double* A, idx lda, idx* P, double* gaps)
+{ return getrf_impl<std::complex<double>>(m, n, reinterpret_cast<std::complex<double>*>(A), lda, P, gaps); }
+int orc_getrs_z(int trans, idx n, idx nrhs, const double* LU, idx lda, const idx* P, double* B,
+                idx ldb)
+{
+    typedef std::complex<double> Z;
+    if (n <= 0 || nrhs <= 0) return 0;
+    return lu_solve<Z>(trans, make_view(reinterpret_cast<Z*>(const_cast<double*>(LU)), n, n, lda, 0),
+                       P, make_view(reinterpret_cast<Z*>(B), n, nrhs, ldb, 0));
+}
 
 // Determinant from the factors: sign(P) * prod(diag U), the value
 // LUDiv::det returns (src/TMV_LUD.cpp:132-156; Permutation::det at

@@ -187,6 +187,9 @@ int tmvref_gemm_z(int a_rm, int b_rm, int conj_a, int conj_b, ptrdiff_t m, ptrdi
                   const double* B, ptrdiff_t ldb, int add, double* C, ptrdiff_t ldc)
 { return gemm_cplx<double>(a_rm, b_rm, conj_a, conj_b, m, n, k, are, aim, A, lda, B, ldb, add, C, ldc); }
 
+int tmvref_getrf_z(ptrdiff_t n, double* A, ptrdiff_t lda, ptrdiff_t* P)
+{ return getrf<std::complex<double>>(n, reinterpret_cast<std::complex<double>*>(A), lda, P); }
+
 int tmvref_getrf_f64(ptrdiff_t n, double* A, ptrdiff_t lda, ptrdiff_t* P)
 { return getrf<double>(n, A, lda, P); }
 int tmvref_getrf_f32(ptrdiff_t n, float* A, ptrdiff_t lda, ptrdiff_t* P)

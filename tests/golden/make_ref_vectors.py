@@ -67,6 +67,11 @@ def main() -> None:
     A = rng.uniform_matrix(SEED + 121, 37, 41) + 1j * rng.uniform_matrix(SEED + 122, 37, 41)
     B = rng.uniform_matrix(SEED + 123, 41, 29) + 1j * rng.uniform_matrix(SEED + 124, 41, 29)
     v["zgemm_37x29x41_conjA"] = oracle.ref_gemm_complex(A, B, alpha=0.3 + 0.7j, conj_a=True)
+    # Complex LU pivots (TMV_ABS = modulus, first max on ties).
+    for n in (5, 37, 100, 257):
+        Z = rng.uniform_matrix(SEED + 230 + n, n, n) + 1j * rng.uniform_matrix(SEED + 240 + n, n, n)
+        _, P = oracle.getrf_complex(Z, lib=ref)
+        v[f"zlu_{n}_P"] = P
     # LU on uniform [-1,1] inputs without a diagonal shift (pivoting active).
     for n in (1, 2, 3, 5, 37, 64, 100, 130):
         M = rng.uniform_matrix(SEED + 200 + n, n, n)

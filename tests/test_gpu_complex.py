@@ -117,3 +117,97 @@ def test_zgemm_4096_vs_torch(dev):
     R = A @ B
     mag = (A.real.abs() + A.imag.abs()) @ (B.real.abs() + B.imag.abs())
     assert ((Z - R).abs() <= 8 * n * EPS64 * mag).all()
+
+
+# ---------------------------------------------------------------- complex LU ----
+
+def zresidual(A, LU, P):
+    n = A.shape[0]
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    PA = np.array(A, np.complex128)
+    for i, p in enumerate(P):
+        if p != i:
+            PA[[i, p]] = PA[[p, i]]
+    return float(np.abs(PA - L @ U).max() / np.abs(A).max())
+
+
+def gpu_zgetrf(A, dtype=torch.complex128):
+    dA = to_dev(A, dtype)
+    piv, info = tmv_amd.getrf_complex(dA)
+    return dA, dA.cpu().numpy().astype(np.complex128), piv.cpu().numpy(), info
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 37, 64, 65, 100, 257, 700])
+def test_zgetrf_pivots_and_residual(orc, dev, ref_vectors, n):
+    """Pivots equal to the oracle's (and TMV's where the fixtures hold them)
+    where the pivot gap exceeds 64 n eps; P A = L U to 8 n eps."""
+    print(f"zgetrf n={n}", flush=True)
+    if n in (5, 37, 100, 257):
+        A = rng.uniform_matrix(SEED + 230 + n, n, n) + 1j * rng.uniform_matrix(SEED + 240 + n, n, n)
+    else:
+        A = cplx(SEED + 300 + n, n, n)
+    LUo, Po, gaps = orc.getrf_complex(A, gaps=True)
+    _, LU, P, info = gpu_zgetrf(A)
+    assert info == 0
+    ok = gaps > 64 * n * EPS64
+    k0 = n if ok.all() else int(np.argmin(ok))
+    assert np.array_equal(P[:k0], Po[:k0])
+    if f"zlu_{n}_P" in ref_vectors:
+        assert np.array_equal(P, ref_vectors[f"zlu_{n}_P"])
+    assert zresidual(A, LU, P) <= 8 * n * EPS64
+
+
+def test_zgetrf_zero_column_rule(orc, dev):
+    n = 90
+    A = cplx(SEED + 401, n, n)
+    A[:, 11] = 0.0
+    _, LU, P, info = gpu_zgetrf(A)
+    _, Po = orc.getrf_complex(A)
+    assert info == 12 and P[11] == 11
+    assert np.all(LU[11:, 11] == 0)
+    assert np.array_equal(P[:11], Po[:11])
+
+
+@pytest.mark.parametrize("trans", [False, True])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (33, 3), (257, 2), (600, 5)])
+def test_zgetrs(orc, dev, n, nrhs, trans):
+    print(f"zgetrs n={n} nrhs={nrhs} trans={trans}", flush=True)
+    A, B = cplx(SEED + 500 + n, n, n), cplx(SEED + 520 + n, n, nrhs)
+    dA, LU, P, _ = gpu_zgetrf(A)
+    dB = to_dev(B, torch.complex128)
+    tmv_amd.getrs_complex(dA, torch.from_numpy(P).cuda(), dB, trans=trans)
+    X = dB.cpu().numpy()
+    M = A.T if trans else A
+    r = np.abs(B - M @ X).max() / (np.abs(M).max() * np.abs(X).max() * n)
+    assert r <= 8 * EPS64
+    Xo = orc.getrs_complex(LU, P, B, trans=trans)     # oracle on the GPU's factors
+    assert np.abs(X - Xo).max() <= 1e-10 * max(1.0, np.abs(Xo).max())
+
+
+def test_zgetrs_singular_reports(dev):
+    n = 20
+    A = cplx(SEED + 611, n, n)
+    A[:, 4] = 0.0
+    dA, _, P, info = gpu_zgetrf(A)
+    assert info == 5
+    dB = to_dev(cplx(SEED + 613, n, 1), torch.complex128)
+    with pytest.raises(tmv_amd.SingularError):
+        tmv_amd.getrs_complex(dA, torch.from_numpy(P).cuda(), dB)
+
+
+def test_cgetrf_complex64(orc, dev):
+    n = 300
+    A = cplx(SEED + 700, n, n)
+    A32 = A.astype(np.complex64)
+    _, LU, P, info = gpu_zgetrf(A32, torch.complex64)
+    assert info == 0
+    assert zresidual(A32.astype(np.complex128), LU, P) <= 8 * n * EPS32
+    dA = to_dev(A32, torch.complex64)
+    piv, _ = tmv_amd.getrf_complex(dA)
+    B = cplx(SEED + 701, n, 1).astype(np.complex64)
+    dB = to_dev(B, torch.complex64)
+    tmv_amd.getrs_complex(dA, piv, dB)
+    X = dB.cpu().numpy().astype(np.complex128)
+    r = np.abs(B - A32.astype(np.complex128) @ X).max() / (np.abs(A32).max() * np.abs(X).max() * n)
+    assert r <= 8 * EPS32

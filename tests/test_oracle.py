@@ -170,6 +170,72 @@ def test_complex_gemm_integer_exact(orc, ca, cb):
     assert not np.array_equal(Zw, Aop @ Bop)
 
 
+def _zmat(seed, n):
+    return rng.uniform_matrix(seed, n, n) + 1j * rng.uniform_matrix(seed + 10, n, n)
+
+
+def _zresidual(A, LU, P):
+    n = A.shape[0]
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    PA = np.array(A, np.complex128)
+    for i, p in enumerate(P):
+        if p != i:
+            PA[[i, p]] = PA[[p, i]]
+    return float(np.abs(PA - L @ U).max() / np.abs(A).max())
+
+
+def _gepp_modulus(A):
+    """Textbook right-looking GEPP in numpy with TMV's complex pivot rule
+    (largest modulus, first row on ties).  LAPACK's zgetrf ranks by
+    |re| + |im| instead, so it is no pin for complex pivots."""
+    A = np.array(A, np.complex128)
+    n = A.shape[0]
+    P = np.zeros(n, np.int64)
+    for j in range(n):
+        p = j + int(np.argmax(np.abs(A[j:, j])))
+        P[j] = p
+        A[[j, p]] = A[[p, j]]
+        A[j + 1:, j] *= 1.0 / A[j, j]
+        A[j + 1:, j + 1:] -= np.outer(A[j + 1:, j], A[j, j + 1:])
+    return A, P
+
+
+@pytest.mark.parametrize("n", [5, 37, 100, 257])
+def test_complex_lu_pivots_match_tmv(orc, ref_vectors, n):
+    """Complex LU: the same pivot sequence as TMV's (fixtures) and as a
+    textbook modulus-pivoting GEPP on tie-free inputs; P A = L U to 8 n eps."""
+    A = rng.uniform_matrix(SEED + 230 + n, n, n) + 1j * rng.uniform_matrix(SEED + 240 + n, n, n)
+    LU, P, gaps = orc.getrf_complex(A, gaps=True)
+    assert np.array_equal(P, ref_vectors[f"zlu_{n}_P"])
+    assert (gaps > 64 * n * EPS64).all()
+    if n <= 100:
+        _, Pb = _gepp_modulus(A)
+        assert np.array_equal(P, Pb)
+    assert _zresidual(A, LU, P) <= 8 * n * EPS64
+
+
+def test_complex_lu_zero_column_and_solve(orc):
+    """The underflow rule on a complex column, and the solves (both
+    transposes) to 8 n eps backward error; a planted wrong pivot fails."""
+    n = 60
+    A = _zmat(SEED + 77, n)
+    A[:, 7] = 0.0
+    LU, P = orc.getrf_complex(A)
+    assert P[7] == 7 and np.all(LU[7:, 7] == 0)
+    B = _zmat(SEED + 88, n)
+    A2 = _zmat(SEED + 99, n)
+    LU2, P2 = orc.getrf_complex(A2)
+    for trans in (False, True):
+        X = orc.getrs_complex(LU2, P2, B, trans=trans)
+        M = A2.T if trans else A2
+        r = np.abs(B - M @ X).max() / (np.abs(M).max() * np.abs(X).max() * n)
+        assert r <= 8 * EPS64
+    Pw = P2.copy()
+    Pw[3] = (Pw[3] + 1) % n if Pw[3] + 1 < n else 3
+    assert _zresidual(A2, LU2, Pw) > 1e-3
+
+
 def test_inverse_example_golden(orc):
     """examples/Division.cpp:75-133: A2.inverse(), 5./A2, A2.inverse() * b
     and b * A2.inverse() to the 6 printed significant digits."""

@@ -49,6 +49,10 @@ _SIGS = {
     "tmvx_getrs_f32": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _int, _vp]),
     "tmvx_gemm_z": (_int, [_int, _int, _int, _int, _i64, _i64, _i64, _d, _d, _vp, _i64, _vp, _i64, _d, _d, _vp, _i64, _vp]),
     "tmvx_gemm_c": (_int, [_int, _int, _int, _int, _i64, _i64, _i64, _f, _f, _vp, _i64, _vp, _i64, _f, _f, _vp, _i64, _vp]),
+    "tmvx_getrf_z": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
+    "tmvx_getrf_c": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
+    "tmvx_getrs_z": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "tmvx_getrs_c": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tmvx_getri_f64": (_int, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tmvx_getri_f32": (_int, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tmvx_logdet_f64": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
@@ -259,6 +263,37 @@ def getrf(A, stream=None):
     _check(fn(n, A.data_ptr(), lda, piv.data_ptr(), ctypes.byref(info), _stream(stream)),
            "getrf")
     return piv, int(info.value)
+
+
+def getrf_complex(A, stream=None):
+    """In-place PA = LU of a square column-major complex128 / complex64
+    tensor (TMV LU_Decompose for complex T).  Returns (piv, info)."""
+    torch = _torch()
+    n = A.shape[0]
+    if A.shape[1] != n:
+        raise ValueError("square matrices only")
+    lda = _colmajor_ld(A, "A")
+    if A.dtype not in (torch.complex128, torch.complex64):
+        raise TypeError("complex128 or complex64 only")
+    piv = torch.empty(n, dtype=torch.int64, device=A.device)
+    info = _i64(0)
+    fn = lib().tmvx_getrf_z if A.dtype == torch.complex128 else lib().tmvx_getrf_c
+    _check(fn(n, A.data_ptr(), lda, piv.data_ptr(), ctypes.byref(info), _stream(stream)),
+           "getrf_complex")
+    return piv, int(info.value)
+
+
+def getrs_complex(LU, piv, B, trans=False, stream=None):
+    """In-place complex solve with the factors: B := A^-1 B (trans: A^-T B)."""
+    torch = _torch()
+    n = LU.shape[0]
+    lda = _colmajor_ld(LU, "LU")
+    Bm = B.view(n, 1) if B.dim() == 1 else B
+    ldb = _colmajor_ld(Bm, "B") if Bm.shape[1] > 1 else max(n, 1)
+    fn = lib().tmvx_getrs_z if LU.dtype == torch.complex128 else lib().tmvx_getrs_c
+    _check(fn(int(bool(trans)), n, Bm.shape[1], LU.data_ptr(), lda, piv.data_ptr(),
+              Bm.data_ptr(), ldb, _stream(stream)), "getrs_complex")
+    return B
 
 
 def getrf_tournament(A, stream=None):

@@ -86,4 +86,11 @@ int gemm_launch(int a_rowmajor, int b_rowmajor, i64 m, i64 n, i64 k, T alpha,
                 const T* A, i64 lda, const T* B, i64 ldb, T beta, T* C,
                 i64 ldc, hipStream_t s);
 
+// Complex C := (ar + i ai) op(A) op(B) + (br + i bi) C on interleaved
+// complex storage of real type R (tmvx_complex.hip).
+template <typename R>
+int gemm_complex(int la, int lb, int conj_a, int conj_b, i64 m, i64 n, i64 k, R ar, R ai,
+                 const R* A, i64 lda, const R* B, i64 ldb, R br, R bi, R* C, i64 ldc,
+                 hipStream_t s);
+
 } // namespace tmvx

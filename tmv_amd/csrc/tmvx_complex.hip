@@ -129,6 +129,13 @@ int gemm_complex(int la, int lb, int conj_a, int conj_b, i64 m, i64 n, i64 k, R 
     return rc;
 }
 
+template int gemm_complex<double>(int, int, int, int, i64, i64, i64, double, double,
+                                  const double*, i64, const double*, i64, double, double,
+                                  double*, i64, hipStream_t);
+template int gemm_complex<float>(int, int, int, int, i64, i64, i64, float, float,
+                                 const float*, i64, const float*, i64, float, float, float*,
+                                 i64, hipStream_t);
+
 } // namespace tmvx
 
 using namespace tmvx;
