@@ -12,9 +12,9 @@
 //   m.setDiv(), m.lud() -> LUDiv<T> with getL/getU/getP/det/logDet/
 //   isSingular/isTrans                             (TMV_Matrix.h:933-975,
 //                                                   TMV_LUD.h:90-161)
-//   complex products: Matrix<std::complex<double|float>> with *, MultMM
-//                                                  (src/TMV_MultMM_Block.cpp:
-//                                                   334-392)
+//   complex matrices: Matrix<std::complex<double|float>> with *, MultMM,
+//   x = b/m, b%m                                   (src/TMV_MultMM_Block.cpp:
+//                                                   334-392; LU as above)
 //   m.inverse(), m.det(), m.logDet(&sign)          (TMV_Matrix.h:869-871,
 //                                                   926-928,
 //                                                   src/TMV_LUD.cpp:133-169)
@@ -132,8 +132,8 @@ template <> struct Abi<float> {
     { return tmvx_logdet_f32(n, LU, lda, p, ld, sg, nullptr); }
 };
 
-// Complex element types: products only (TMV's complex LU is not on this
-// library's path), through the complex GEMM entry points.
+// Complex element types: products and LU division through the complex
+// entry points (factors and solves; det/logDet/inverse stay real-only).
 template <> struct Abi<std::complex<double>> {
     typedef std::complex<double> Z;
     static int gemm(int la, int lb, int64_t m, int64_t n, int64_t k, Z al, const Z* A,
@@ -143,6 +143,14 @@ template <> struct Abi<std::complex<double>> {
                            reinterpret_cast<const double*>(A), lda,
                            reinterpret_cast<const double*>(B), ldb, be.real(), be.imag(),
                            reinterpret_cast<double*>(C), ldc, nullptr);
+    }
+    static int getrf(int64_t n, Z* A, int64_t lda, int64_t* piv, int64_t* info)
+    { return tmvx_getrf_z(n, reinterpret_cast<double*>(A), lda, piv, info, nullptr); }
+    static int getrs(int tr, int64_t n, int64_t nr, const Z* LU, int64_t lda, const int64_t* p,
+                     Z* B, int64_t ldb)
+    {
+        return tmvx_getrs_z(tr, n, nr, reinterpret_cast<const double*>(LU), lda, p,
+                            reinterpret_cast<double*>(B), ldb, nullptr);
     }
 };
 template <> struct Abi<std::complex<float>> {
@@ -154,6 +162,14 @@ template <> struct Abi<std::complex<float>> {
                            reinterpret_cast<const float*>(A), lda,
                            reinterpret_cast<const float*>(B), ldb, be.real(), be.imag(),
                            reinterpret_cast<float*>(C), ldc, nullptr);
+    }
+    static int getrf(int64_t n, Z* A, int64_t lda, int64_t* piv, int64_t* info)
+    { return tmvx_getrf_c(n, reinterpret_cast<float*>(A), lda, piv, info, nullptr); }
+    static int getrs(int tr, int64_t n, int64_t nr, const Z* LU, int64_t lda, const int64_t* p,
+                     Z* B, int64_t ldb)
+    {
+        return tmvx_getrs_c(tr, n, nr, reinterpret_cast<const float*>(LU), lda, p,
+                            reinterpret_cast<float*>(B), ldb, nullptr);
     }
 };
 

@@ -254,6 +254,28 @@ static void TestComplexProduct()
                                    : "complex MultMM<true> (row-major C)");
 }
 
+// Complex LU division: x = b/m and x = b%m for a complex 3x3 with integer
+// parts; A x reproduces b to a few eps.
+static void TestComplexDivision()
+{
+    typedef std::complex<double> Z;
+    tmv::Matrix<Z> m(3, 3);
+    const Z v[9] = {Z(2, 1), Z(0, -3), Z(1, 1), Z(-1, 2), Z(4, 0), Z(0, 1), Z(3, -2), Z(1, 1), Z(-2, 5)};
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) m(i, j) = v[i * 3 + j];
+    tmv::Vector<Z> b(3);
+    b(0) = Z(1, 0); b(1) = Z(0, 2); b(2) = Z(-3, 1);
+    tmv::Vector<Z> x = b / m;
+    tmv::Vector<Z> y = b % m;
+    double e1 = 0, e2 = 0;
+    for (int i = 0; i < 3; ++i) {
+        Z s1(0, 0), s2(0, 0);
+        for (int j = 0; j < 3; ++j) { s1 += m(i, j) * x(j); s2 += y(j) * m(j, i); }
+        e1 = std::fmax(e1, std::abs(s1 - b(i)));
+        e2 = std::fmax(e2, std::abs(s2 - b(i)));
+    }
+    Assert(e1 < 1e3 * EPS && e2 < 1e3 * EPS, "complex b/m and b%m");
+}
+
 static void TestMixed()
 {
     const int N = 300;
@@ -288,6 +310,7 @@ int main()
     TestInverse<tmv::RowMajor>();
     TestComplexProduct<tmv::ColMajor>();
     TestComplexProduct<tmv::RowMajor>();
+    TestComplexDivision();
     std::printf("%s: %d failure(s)\n", nfail ? "FAILED" : "PASSED", nfail);
     return nfail ? 1 : 0;
 }
