@@ -154,6 +154,33 @@ int tmvx_getrs_c(int trans, int64_t n, int64_t nrhs, const float* LU, int64_t ld
                  const int64_t* piv, float* B, int64_t ldb, void* stream);
 
 /*
+ * QR decomposition by Householder reflections, in place: A (m x n, m >= n,
+ * column-major) gets R on and above the diagonal and the reflectors below
+ * it (first elements 1, implicit); beta (device, n) the reflectors' scales,
+ * H_j = I - beta_j v_j v_j^T, A = H_0 H_1 ... H_{n-1} R.  Reflectors as
+ * TMV's HouseholderReflect builds them (R(j,j) = -sign(x0) |x|, zero or
+ * underflowing columns give beta = 0).  Functional, not tuned.
+ * Replaces: tmv::QR_Decompose(MatrixView A, VectorView beta, T& det)
+ * (src/TMV_QRDecompose.cpp:603-650; reflector src/TMV_Householder.cpp:46-180).
+ */
+int tmvx_geqrf_f64(int64_t m, int64_t n, double* A, int64_t lda, double* beta, void* stream);
+int tmvx_geqrf_f32(int64_t m, int64_t n, float* A, int64_t lda, float* beta, void* stream);
+
+/*
+ * QR division with the factors of tmvx_geqrf_*: B (m x nrhs) := Q^T B, then
+ * its first n rows := R^-1 of themselves -- the least-squares solution of
+ * A X = B for m > n, the solution for m == n (rows n..m-1 of B keep the
+ * residual's Q^T components).  A zero diagonal of R reports
+ * TMVX_ERR_SINGULAR.  Synchronises.
+ * Replaces: tmv::QR_LDiv (src/TMV_QRDiv.cpp:33-94) behind x = b/m with
+ * m.divideUsing(tmv::QR).
+ */
+int tmvx_geqrs_f64(int64_t m, int64_t n, int64_t nrhs, const double* QR, int64_t lda,
+                   const double* beta, double* B, int64_t ldb, void* stream);
+int tmvx_geqrs_f32(int64_t m, int64_t n, int64_t nrhs, const float* QR, int64_t lda,
+                   const float* beta, float* B, int64_t ldb, void* stream);
+
+/*
  * X := A^-1 from the factors of tmvx_getrf_* (X n x n column-major, not
  * aliasing LU).  The same matrix TMV's LU_Inverse returns (U^-1 L^-1 with
  * the columns reverse-permuted by piv); formed here as the solve of

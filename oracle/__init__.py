@@ -20,6 +20,7 @@ import os
 import subprocess
 
 import numpy as np
+import scipy.linalg as sla
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
@@ -246,6 +247,102 @@ def getrs_complex(LU, P, B, trans=False):
     if rc != 0:
         raise RuntimeError(f"complex getrs failed: {rc}")
     return X[:, 0] if vec else X
+
+
+def householder_reflect(x):
+    """TMV's HouseholderReflect (src/TMV_Householder.cpp:46-180) on a real
+    vector x (x[0] = x0), in place: returns beta; x[0] becomes y*scale and
+    x[1:] the reflector (its implicit first element 1)."""
+    rmin = np.finfo(x.dtype).tiny
+    tail = x[1:]
+    scale = max(float(np.abs(tail).max(initial=0.0)), abs(float(x[0])))
+    if scale < rmin:                       # hopeless: zero the vector
+        x[:] = 0.0
+        return x.dtype.type(0)
+    invscale = 1.0 / scale
+    normsqx = float(np.sum((tail * invscale) ** 2))
+    if normsqx == 0.0:                     # H = I
+        tail[:] = 0.0
+        return x.dtype.type(0)
+    x0 = float(x[0]) * invscale
+    normsq = normsqx + x0 * x0
+    normx = np.sqrt(normsq)
+    y = -normx if x0 > 0 else normx
+    v0 = x0 - y
+    beta = (v0 * v0) / (normsq - y * x0)
+    tail *= (1.0 / v0) * invscale
+    x[0] = y * scale
+    return x.dtype.type(beta)
+
+
+def qr_decompose(A, dtype=np.float64):
+    """TMV's NonBlockQRDecompose (src/TMV_QRDecompose.cpp:55-110): per
+    column the reflector (householder_reflect), then H = I - beta v v^T on
+    the columns to its right (HouseholderLMult, src/TMV_Householder.cpp:
+    333-390).  Returns (QR packed like TMV's, beta)."""
+    A = np.array(A, dtype, order="F")
+    m, n = A.shape
+    beta = np.zeros(n, dtype)
+    for j in range(n):
+        b = householder_reflect(A[j:, j])
+        beta[j] = b
+        if b != 0 and j + 1 < n:
+            v = np.concatenate(([1.0], A[j + 1:, j])).astype(dtype)
+            w = v @ A[j:, j + 1:]
+            A[j:, j + 1:] -= b * np.outer(v, w)
+    return A, beta
+
+
+def qr_solve(QR, beta, B):
+    """QR_LDiv (src/TMV_QRDiv.cpp:33-94): y = Q^T b (the reflectors in
+    order), then x = R^-1 y(0:n)."""
+    QR = np.asarray(QR)
+    m, n = QR.shape
+    Y = np.array(B, np.float64).reshape(m, -1)
+    for j in range(n):
+        if beta[j] != 0:
+            v = np.concatenate(([1.0], QR[j + 1:, j]))
+            Y[j:] -= beta[j] * np.outer(v, v @ Y[j:])
+    R = np.triu(QR[:n, :n])
+    if np.any(np.diag(R) == 0):
+        raise SingularError("singular R")
+    X = sla.solve_triangular(R, Y[:n], lower=False)
+    return X[:, 0] if np.asarray(B).ndim == 1 else X
+
+
+def ref_geqrf(A):
+    """QR_Decompose through the real TMV: (QR, beta)."""
+    lib = ref_lib()
+    if lib is None:
+        raise FileNotFoundError(REF_SO)
+    A = np.array(A, np.float64, order="F")
+    m, n = A.shape
+    beta = np.zeros(n)
+    f = lib.fn("geqrf_f64")
+    f.argtypes = [_i64, _i64, _vp, _i64, _vp]
+    if f(m, n, _ptr(A), max(1, m), _ptr(beta)) != 0:
+        raise RuntimeError("geqrf failed")
+    return A, beta
+
+
+def ref_qr_solve(A, B):
+    """x = b/A with QR division through the real TMV."""
+    lib = ref_lib()
+    if lib is None:
+        raise FileNotFoundError(REF_SO)
+    A = np.asfortranarray(A, np.float64)
+    m, n = A.shape
+    Bv = np.asarray(B, np.float64)
+    Bm = np.asfortranarray(Bv.reshape(m, -1))
+    X = np.zeros((n, Bm.shape[1]), order="F")
+    f = lib.fn("qr_solve_f64")
+    f.argtypes = [_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64]
+    rc = f(m, n, Bm.shape[1], _ptr(A), max(1, m), _ptr(Bm), max(1, m), _ptr(X), max(1, n))
+    if rc == -2:
+        raise SingularError("singular")
+    if rc != 0:
+        raise RuntimeError(f"qr solve failed: {rc}")
+    return X[:, 0] if Bv.ndim == 1 else X
 
 
 def getrs(LU, P, B, trans=False, dtype=np.float64, lib: _Lib | None = None):

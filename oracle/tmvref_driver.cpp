@@ -11,6 +11,7 @@
 // boundary: they become negative return codes.
 #include "TMV.h"
 #include "TMV_LUDiv.h"   // LU_LDivEq / LU_RDivEq (src/TMV_LUDiv.h:29-35)
+#include "TMV_QRDiv.h"   // QR_Decompose(A, beta, det) (src/TMV_QRDiv.h:34-41)
 #include <cstring>
 #include <omp.h>
 
@@ -65,6 +66,35 @@ int gemm_cplx(int a_rm, int b_rm, int conj_a, int conj_b, ptrdiff_t m, ptrdiff_t
         tmv::MatrixView<CT> Cv = mview(reinterpret_cast<CT*>(C), m, n, ldc, 0);
         if (add) tmv::MultMM<true>(CT(are, aim), Av, Bv, Cv);
         else tmv::MultMM<false>(CT(are, aim), Av, Bv, Cv);
+    } catch (...) { return -1; }
+    return 0;
+}
+
+// QR through tmv::QR_Decompose(A, beta, det) (src/TMV_QRDecompose.cpp:603-650).
+template <class T>
+int geqrf(ptrdiff_t m, ptrdiff_t n, T* A, ptrdiff_t lda, T* beta)
+{
+    try {
+        T det(1);
+        tmv::QR_Decompose(mview(A, m, n, lda, 0), tmv::VectorView<T>(beta, n, 1, tmv::NonConj), det);
+    } catch (...) { return -1; }
+    return 0;
+}
+
+// x = b/A with QR division (least squares for m > n), src/TMV_QRDiv.cpp:33-94.
+template <class T>
+int qr_solve(ptrdiff_t m, ptrdiff_t n, ptrdiff_t nrhs, const T* A, ptrdiff_t lda, const T* B,
+             ptrdiff_t ldb, T* X, ptrdiff_t ldx)
+{
+    try {
+        tmv::Matrix<T, tmv::ColMajor> Am(m, n);
+        Am = cview(A, m, n, lda, 0);
+        Am.divideUsing(tmv::QR);
+        tmv::Matrix<T, tmv::ColMajor> Bm(m, nrhs);
+        Bm = cview(B, m, nrhs, ldb, 0);
+        tmv::Matrix<T, tmv::ColMajor> Xm = Bm / Am;
+        mview(X, n, nrhs, ldx, 0) = Xm;
+    } catch (tmv::Singular&) { return -2;
     } catch (...) { return -1; }
     return 0;
 }
@@ -189,6 +219,12 @@ int tmvref_gemm_z(int a_rm, int b_rm, int conj_a, int conj_b, ptrdiff_t m, ptrdi
 
 int tmvref_getrf_z(ptrdiff_t n, double* A, ptrdiff_t lda, ptrdiff_t* P)
 { return getrf<std::complex<double>>(n, reinterpret_cast<std::complex<double>*>(A), lda, P); }
+
+int tmvref_geqrf_f64(ptrdiff_t m, ptrdiff_t n, double* A, ptrdiff_t lda, double* beta)
+{ return geqrf<double>(m, n, A, lda, beta); }
+int tmvref_qr_solve_f64(ptrdiff_t m, ptrdiff_t n, ptrdiff_t nrhs, const double* A, ptrdiff_t lda,
+                        const double* B, ptrdiff_t ldb, double* X, ptrdiff_t ldx)
+{ return qr_solve<double>(m, n, nrhs, A, lda, B, ldb, X, ldx); }
 
 int tmvref_getrf_f64(ptrdiff_t n, double* A, ptrdiff_t lda, ptrdiff_t* P)
 { return getrf<double>(n, A, lda, P); }

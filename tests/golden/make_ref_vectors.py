@@ -72,6 +72,14 @@ def main() -> None:
         Z = rng.uniform_matrix(SEED + 230 + n, n, n) + 1j * rng.uniform_matrix(SEED + 240 + n, n, n)
         _, P = oracle.getrf_complex(Z, lib=ref)
         v[f"zlu_{n}_P"] = P
+    # QR (QR_Decompose's beta and R) and QR division, tall and square.
+    for (m, n) in ((37, 37), (100, 64)):
+        Q = rng.uniform_matrix(SEED + 260 + m + n, m, n)
+        QR, beta = oracle.ref_geqrf(Q)
+        v[f"qr_{m}x{n}_beta"] = beta
+        v[f"qr_{m}x{n}_R"] = np.triu(QR[:n])
+        Bq = rng.uniform_matrix(SEED + 270 + m, m, 2)
+        v[f"qr_{m}x{n}_x"] = oracle.ref_qr_solve(Q, Bq)
     # LU on uniform [-1,1] inputs without a diagonal shift (pivoting active).
     for n in (1, 2, 3, 5, 37, 64, 100, 130):
         M = rng.uniform_matrix(SEED + 200 + n, n, n)

@@ -236,6 +236,61 @@ def test_complex_lu_zero_column_and_solve(orc):
     assert _zresidual(A2, LU2, Pw) > 1e-3
 
 
+# ------------------------------------------------------------------ QR ----
+
+def _q_from(QR, beta):
+    """Q = H_0 H_1 ... H_{n-1} applied to the identity (m x m)."""
+    m, n = QR.shape
+    Q = np.eye(m)
+    for j in reversed(range(n)):
+        v = np.concatenate(([1.0], QR[j + 1:, j]))
+        Q[j:] -= beta[j] * np.outer(v, v @ Q[j:])
+    return Q
+
+
+@pytest.mark.parametrize("m,n", [(37, 37), (100, 64)])
+def test_qr_matches_tmv_and_lapack(orc, ref_vectors, m, n):
+    """The restatement of TMV's NonBlockQRDecompose + HouseholderReflect:
+    beta and R equal TMV's (fixtures) and LAPACK dgeqrf's (the same
+    reflector convention) to rounding; A = Q R to 8 m eps; QR division
+    equals TMV's x = b/A and numpy's least squares."""
+    A = rng.uniform_matrix(SEED + 260 + m + n, m, n)
+    QR, beta = orc.qr_decompose(A)
+    tol = 64 * m * EPS64 * np.abs(A).max()
+    assert np.abs(beta - ref_vectors[f"qr_{m}x{n}_beta"]).max() <= 64 * m * EPS64
+    assert np.abs(np.triu(QR[:n]) - ref_vectors[f"qr_{m}x{n}_R"]).max() <= tol
+    lq, ltau = sla.lapack.dgeqrf(A)[:2]
+    assert np.abs(np.triu(QR[:n]) - np.triu(lq[:n])).max() <= tol
+    assert np.abs(beta - ltau).max() <= 64 * m * EPS64
+    Q = _q_from(QR, beta)
+    R = np.zeros((m, n))
+    R[:n] = np.triu(QR[:n])
+    assert np.abs(A - Q @ R).max() <= 8 * m * EPS64 * np.abs(A).max()
+    B = rng.uniform_matrix(SEED + 270 + m, m, 2)
+    X = orc.qr_solve(QR, beta, B)
+    np.testing.assert_allclose(X, ref_vectors[f"qr_{m}x{n}_x"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(X, np.linalg.lstsq(A, B, rcond=None)[0], rtol=0, atol=1e-12)
+    # a reflector with the wrong sign convention breaks the reconstruction
+    Rw = R.copy()
+    Rw[0] = -Rw[0]
+    assert np.abs(A - Q @ Rw).max() > 1e-3
+
+
+def test_qr_zero_column_and_singular(orc):
+    """A column already zero below (and on) the diagonal gets beta = 0 and
+    R(j, j) = 0 (HouseholderReflect's no-reflection cases); QR division then
+    reports a singular R."""
+    A = rng.uniform_matrix(SEED + 281, 40, 12)
+    A[:, 5] = 0.0
+    QR, beta = orc.qr_decompose(A)
+    ref = orc.ref_geqrf(A) if orc.ref_lib() is not None else None
+    assert QR[5, 5] == 0.0
+    if ref is not None:
+        assert abs(ref[0][5, 5]) <= 1e-12 and abs(beta[5] - ref[1][5]) <= 1e-12
+    with pytest.raises(orc.SingularError):
+        orc.qr_solve(QR, beta, np.ones(40))
+
+
 def test_inverse_example_golden(orc):
     """examples/Division.cpp:75-133: A2.inverse(), 5./A2, A2.inverse() * b
     and b * A2.inverse() to the 6 printed significant digits."""

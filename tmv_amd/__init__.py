@@ -53,6 +53,10 @@ _SIGS = {
     "tmvx_getrf_c": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
     "tmvx_getrs_z": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tmvx_getrs_c": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "tmvx_geqrf_f64": (_int, [_i64, _i64, _vp, _i64, _vp, _vp]),
+    "tmvx_geqrf_f32": (_int, [_i64, _i64, _vp, _i64, _vp, _vp]),
+    "tmvx_geqrs_f64": (_int, [_i64, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "tmvx_geqrs_f32": (_int, [_i64, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tmvx_getri_f64": (_int, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tmvx_getri_f32": (_int, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tmvx_logdet_f64": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
@@ -293,6 +297,31 @@ def getrs_complex(LU, piv, B, trans=False, stream=None):
     fn = lib().tmvx_getrs_z if LU.dtype == torch.complex128 else lib().tmvx_getrs_c
     _check(fn(int(bool(trans)), n, Bm.shape[1], LU.data_ptr(), lda, piv.data_ptr(),
               Bm.data_ptr(), ldb, _stream(stream)), "getrs_complex")
+    return B
+
+
+def geqrf(A, stream=None):
+    """In-place QR (Householder) of a column-major m x n tensor, m >= n
+    (TMV QR_Decompose).  Returns beta (device, n)."""
+    torch = _torch()
+    m, n = A.shape
+    lda = _colmajor_ld(A, "A") if n > 1 else max(m, 1)
+    beta = torch.empty(n, dtype=A.dtype, device=A.device)
+    fn = getattr(lib(), "tmvx_geqrf_" + _suffix(A.dtype))
+    _check(fn(m, n, A.data_ptr(), lda, beta.data_ptr(), _stream(stream)), "geqrf")
+    return beta
+
+
+def geqrs(QR, beta, B, stream=None):
+    """QR division: B (m x nrhs) := Q^T B, then its first n rows := R^-1 of
+    themselves (least squares for m > n).  Returns B."""
+    m, n = QR.shape
+    lda = _colmajor_ld(QR, "QR") if n > 1 else max(m, 1)
+    Bm = B.view(m, 1) if B.dim() == 1 else B
+    ldb = _colmajor_ld(Bm, "B") if Bm.shape[1] > 1 else max(m, 1)
+    fn = getattr(lib(), "tmvx_geqrs_" + _suffix(QR.dtype))
+    _check(fn(m, n, Bm.shape[1], QR.data_ptr(), lda, beta.data_ptr(), Bm.data_ptr(), ldb,
+              _stream(stream)), "geqrs")
     return B
 
 
