@@ -8,7 +8,7 @@
 //                                                  (TMV_MatrixArith.h:700-812,
 //                                                   TMV_MatrixArithFunc.h:53-57)
 //   x = b/m, X = B/m (LDiv), x = b%m (RDiv)        (TMV_MatrixArith.h:979-1177)
-//   m.divideUsing(tmv::LU), m.saveDiv(), m.resetDiv(), m.unsetDiv(),
+//   m.divideUsing(tmv::LU | tmv::QR), m.saveDiv(), m.resetDiv(), m.unsetDiv(),
 //   m.setDiv(), m.lud() -> LUDiv<T> with getL/getU/getP/det/logDet/
 //   isSingular/isTrans                             (TMV_Matrix.h:933-975,
 //                                                   TMV_LUD.h:90-161)
@@ -47,7 +47,7 @@
 namespace tmv {
 
 enum StorageType { ColMajor = 0, RowMajor = 2 };
-enum DivType { XX = 0, LU = 1 };
+enum DivType { XX = 0, LU = 1, QR = 4 };    // TMV's values (TMV_Base.h:209-210)
 
 class Error : public std::runtime_error {
 public:
@@ -108,6 +108,11 @@ template <> struct Abi<double> {
     static int getrs(int tr, int64_t n, int64_t nr, const double* LU, int64_t lda, const int64_t* p,
                      double* B, int64_t ldb)
     { return tmvx_getrs_f64(tr, n, nr, LU, lda, p, B, ldb, 1, nullptr); }
+    static int geqrf(int64_t m, int64_t n, double* A, int64_t lda, double* beta)
+    { return tmvx_geqrf_f64(m, n, A, lda, beta, nullptr); }
+    static int geqrs(int64_t m, int64_t n, int64_t nr, const double* QR, int64_t lda,
+                     const double* beta, double* B, int64_t ldb)
+    { return tmvx_geqrs_f64(m, n, nr, QR, lda, beta, B, ldb, nullptr); }
     static int getri(int64_t n, const double* LU, int64_t lda, const int64_t* p, double* X,
                      int64_t ldx)
     { return tmvx_getri_f64(n, LU, lda, p, X, ldx, nullptr); }
@@ -124,6 +129,11 @@ template <> struct Abi<float> {
     static int getrs(int tr, int64_t n, int64_t nr, const float* LU, int64_t lda, const int64_t* p,
                      float* B, int64_t ldb)
     { return tmvx_getrs_f32(tr, n, nr, LU, lda, p, B, ldb, 1, nullptr); }
+    static int geqrf(int64_t m, int64_t n, float* A, int64_t lda, float* beta)
+    { return tmvx_geqrf_f32(m, n, A, lda, beta, nullptr); }
+    static int geqrs(int64_t m, int64_t n, int64_t nr, const float* QR, int64_t lda,
+                     const float* beta, float* B, int64_t ldb)
+    { return tmvx_geqrs_f32(m, n, nr, QR, lda, beta, B, ldb, nullptr); }
     static int getri(int64_t n, const float* LU, int64_t lda, const int64_t* p, float* X,
                      int64_t ldx)
     { return tmvx_getri_f32(n, LU, lda, p, X, ldx, nullptr); }
@@ -146,6 +156,9 @@ template <> struct Abi<std::complex<double>> {
     }
     static int getrf(int64_t n, Z* A, int64_t lda, int64_t* piv, int64_t* info)
     { return tmvx_getrf_z(n, reinterpret_cast<double*>(A), lda, piv, info, nullptr); }
+    static int geqrf(int64_t, int64_t, Z*, int64_t, Z*) { return TMVX_ERR_UNSUPPORTED; }
+    static int geqrs(int64_t, int64_t, int64_t, const Z*, int64_t, const Z*, Z*, int64_t)
+    { return TMVX_ERR_UNSUPPORTED; }
     static int getrs(int tr, int64_t n, int64_t nr, const Z* LU, int64_t lda, const int64_t* p,
                      Z* B, int64_t ldb)
     {
@@ -165,6 +178,9 @@ template <> struct Abi<std::complex<float>> {
     }
     static int getrf(int64_t n, Z* A, int64_t lda, int64_t* piv, int64_t* info)
     { return tmvx_getrf_c(n, reinterpret_cast<float*>(A), lda, piv, info, nullptr); }
+    static int geqrf(int64_t, int64_t, Z*, int64_t, Z*) { return TMVX_ERR_UNSUPPORTED; }
+    static int geqrs(int64_t, int64_t, int64_t, const Z*, int64_t, const Z*, Z*, int64_t)
+    { return TMVX_ERR_UNSUPPORTED; }
     static int getrs(int tr, int64_t n, int64_t nr, const Z* LU, int64_t lda, const int64_t* p,
                      Z* B, int64_t ldb)
     {
@@ -210,6 +226,7 @@ private:
 };
 
 template <class T> class LUDiv;
+template <class T> class QRDiv;
 
 template <class T, int S = ColMajor> class Matrix {
 public:
@@ -249,17 +266,32 @@ public:
     // ---- division control (TMV_Matrix.h:933-975, TMV_BaseMatrix.cpp) ----
     void divideUsing(DivType dt)
     {
-        if (dt != LU) throw Error("only LU division is provided by this library");
+        if (dt != LU && dt != QR && dt != XX)
+            throw Error("only LU and QR division are provided by this library");
         if (dt != divtype_) unsetDiv();
         divtype_ = dt;
     }
+    // XX resolves to LU for a square matrix, QR otherwise
+    // (DivHelper::resetDivType, src/TMV_BaseMatrix.cpp:234-236).
+    bool usesQR() const { return divtype_ == QR || (divtype_ == XX && !isSquare()); }
     void saveDiv() { savediv_ = true; }
-    void setDiv() const { if (!div_) div_.reset(new LUDiv<T>(*this)); }
-    void unsetDiv() const { div_.reset(); }
+    void setDiv() const
+    {
+        if (usesQR()) { if (!qdiv_) qdiv_.reset(new QRDiv<T>(*this)); }
+        else if (!div_) div_.reset(new LUDiv<T>(*this));
+    }
+    void unsetDiv() const { div_.reset(); qdiv_.reset(); }
     void resetDiv() const { unsetDiv(); }
-    bool divIsSet() const { return (bool)div_; }
+    bool divIsSet() const { return (bool)div_ || (bool)qdiv_; }
+    QRDiv<T>& qrd() const
+    {
+        if (!usesQR()) throw Error("qrd(): the division type is not QR");
+        setDiv();
+        return *qdiv_;
+    }
     LUDiv<T>& lud() const
     {
+        if (usesQR()) throw Error("lud(): the division type is QR");
         setDiv();
         return *div_;
     }
@@ -270,6 +302,7 @@ public:
     T det() const
     {
         if (m_ == 0) return T(1);
+        if (usesQR()) throw Error("det() is provided with LU division");
         setDiv();
         const T d = div_->det();
         doneDiv();
@@ -278,6 +311,7 @@ public:
     T logDet(T* sign = nullptr) const
     {
         if (m_ == 0) { if (sign) *sign = T(1); return T(0); }
+        if (usesQR()) throw Error("logDet() is provided with LU division");
         setDiv();
         const T ld = div_->logDet(sign);
         doneDiv();
@@ -288,6 +322,7 @@ public:
     Matrix inverse() const
     {
         if (!isSquare()) throw Error("inverse of a non-square matrix needs QR or SV division");
+        if (usesQR()) throw Error("inverse() is provided with LU division");
         Matrix R(m_, n_);
         if (m_ == 0) return R;
         setDiv();
@@ -299,9 +334,61 @@ public:
 private:
     ptrdiff_t m_, n_;
     std::vector<T> a_;
-    DivType divtype_ = LU;
+    DivType divtype_ = XX;
     bool savediv_ = false;
     mutable std::shared_ptr<LUDiv<T>> div_;
+    mutable std::shared_ptr<QRDiv<T>> qdiv_;
+};
+
+// ---------------------------------------------------------------------------
+// QR divider (QRDiv, src/TMV_QRD.cpp; QR_LDiv, src/TMV_QRDiv.cpp:33-94):
+// A (m x n, m >= n) factored on the device; x = b/A is the least-squares
+// solution for m > n.
+// ---------------------------------------------------------------------------
+template <class T> class QRDiv {
+public:
+    template <int S>
+    explicit QRDiv(const Matrix<T, S>& A) : m_(A.colsize()), n_(A.rowsize())
+    {
+        if (m_ < n_) throw Error("QR division needs colsize >= rowsize");
+        qr_.resize((size_t)(m_ * n_));
+        beta_.resize((size_t)n_);
+        if (n_ == 0) return;
+        const Matrix<T, ColMajor> Ac(A);
+        qr_.upload(Ac.cptr(), (size_t)(m_ * n_));
+        detail::check(detail::Abi<T>::geqrf(m_, n_, qr_.get(), m_, beta_.get()), "QR_Decompose");
+    }
+    // x (n x nr, column-major) from b (m x nr, column-major).
+    void lDivCM(const T* b, T* x, ptrdiff_t nr) const
+    {
+        if (n_ == 0 || nr == 0) return;
+        detail::DevBuf<T> bb((size_t)(m_ * nr));
+        bb.upload(b, (size_t)(m_ * nr));
+        detail::check(detail::Abi<T>::geqrs(m_, n_, nr, qr_.get(), m_, beta_.get(), bb.get(), m_),
+                      "QR_LDiv");
+        std::vector<T> h((size_t)(m_ * nr));
+        bb.download(h.data(), (size_t)(m_ * nr));
+        for (ptrdiff_t j = 0; j < nr; ++j)
+            for (ptrdiff_t i = 0; i < n_; ++i) x[i + j * n_] = h[(size_t)(i + j * m_)];
+    }
+    // Packed factors: R on and above the diagonal, reflectors below.
+    Matrix<T, ColMajor> getQR() const
+    {
+        Matrix<T, ColMajor> M(m_, n_);
+        if (m_ * n_) qr_.download(M.ptr(), (size_t)(m_ * n_));
+        return M;
+    }
+    std::vector<T> getBeta() const
+    {
+        std::vector<T> b((size_t)n_);
+        if (n_) beta_.download(b.data(), (size_t)n_);
+        return b;
+    }
+
+private:
+    ptrdiff_t m_, n_;
+    detail::DevBuf<T> qr_;
+    detail::DevBuf<T> beta_;
 };
 
 // ---------------------------------------------------------------------------
@@ -465,6 +552,12 @@ template <class T, int S>
 Vector<T> operator/(const Vector<T>& b, const Matrix<T, S>& m)
 {
     if (b.size() != m.colsize()) throw Error("b/m: size mismatch");
+    if (m.usesQR()) {
+        Vector<T> x(m.rowsize());
+        m.qrd().lDivCM(b.cptr(), x.ptr(), 1);
+        m.doneDiv();
+        return x;
+    }
     Vector<T> x = b;
     m.lud().lDivEqCM(x.ptr(), 1);
     m.doneDiv();
@@ -485,6 +578,13 @@ template <class T, int SB, int S>
 Matrix<T, ColMajor> operator/(const Matrix<T, SB>& B, const Matrix<T, S>& m)
 {
     if (B.colsize() != m.colsize()) throw Error("B/m: size mismatch");
+    if (m.usesQR()) {
+        const Matrix<T, ColMajor> Bc(B);
+        Matrix<T, ColMajor> X(m.rowsize(), B.rowsize());
+        m.qrd().lDivCM(Bc.cptr(), X.ptr(), B.rowsize());
+        m.doneDiv();
+        return X;
+    }
     Matrix<T, ColMajor> X(B);
     m.lud().lDivEqCM(X.ptr(), X.rowsize());
     m.doneDiv();

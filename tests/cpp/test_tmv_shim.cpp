@@ -276,6 +276,25 @@ static void TestComplexDivision()
     Assert(e1 < 1e3 * EPS && e2 < 1e3 * EPS, "complex b/m and b%m");
 }
 
+// QR division: the default for a non-square matrix (least squares), and on
+// request for a square one (same x as LU).
+static void TestQRDivision()
+{
+    tmv::Matrix<double> a(4, 2, {1, 0, 0, 1, 1, 1, 1, -1});
+    tmv::Vector<double> b{1, 2, 3, 4};
+    tmv::Vector<double> x = b / a;        // A^T A = 3 I, A^T b = (8, 1)
+    Assert(std::fabs(x(0) - 8.0 / 3) < 1e3 * EPS && std::fabs(x(1) - 1.0 / 3) < 1e3 * EPS,
+           "QR least squares (default for non-square)");
+    tmv::Matrix<double> m(3, 3, {4, 1, 2, 1, 5, 3, 2, 3, 6});
+    tmv::Vector<double> c{1, -2, 3};
+    const tmv::Vector<double> xl = c / m;
+    m.divideUsing(tmv::QR);
+    const tmv::Vector<double> xq = c / m;
+    double d = 0;
+    for (int i = 0; i < 3; ++i) d = std::fmax(d, std::fabs(xl(i) - xq(i)));
+    Assert(d < 1e3 * EPS, "QR division of a square matrix agrees with LU");
+}
+
 static void TestMixed()
 {
     const int N = 300;
@@ -311,6 +330,7 @@ int main()
     TestComplexProduct<tmv::ColMajor>();
     TestComplexProduct<tmv::RowMajor>();
     TestComplexDivision();
+    TestQRDivision();
     std::printf("%s: %d failure(s)\n", nfail ? "FAILED" : "PASSED", nfail);
     return nfail ? 1 : 0;
 }
