@@ -306,6 +306,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-lowp", dest="lowp", action="store_false")
+    ap.add_argument("--no-grid", dest="grid", action="store_false",
+                    help="skip the kappa x nrhs grid of mixed solves")
     ap.add_argument("--only-batched", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -369,8 +371,49 @@ def main():
                           "backward_error": backward_error_gpu(A2, Xl, b, torch),
                           "factor_ms": round(inf["factor_ms"], 3),
                           "refine_ms": round(inf["refine_ms"], 3)}
-        del A2, Xl
-        torch.cuda.empty_cache()
+        del Xl
+    else:
+        A2 = None
+
+    # ---- the configuration grid: kappa in {1e2, 1e4, 1e6} x nrhs in {1, 64},
+    #      FP32 factorisation + FP64 refinement, one untimed-warm run each ---
+    grid = None
+    if a.grid:
+        grid = {}
+        b64 = torch.empty(n * 64, dtype=torch.float64, device="cuda")
+        tmv_amd.fill_normal(b64, SEED + 16)
+        b64 = b64.view(64, n).t()                 # column-major n x 64
+        b1 = b if nrhs == 1 else b[:, 0].contiguous()
+        mats = [(1e2, A2 if A2 is not None else conditioned_gpu(n, 1e2, SEED + 14, torch, tmv_amd)),
+                (a.kappa, A), (1e6, conditioned_gpu(n, 1e6, SEED + 18, torch, tmv_amd))]
+        for kappa, M in mats:
+            for nr, rhs in ((1, b1), (64, b64)):
+                tmv_amd.gesv_mixed(M, rhs, tol=1e-12, max_iter=50, check=False)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                Xg, inf = tmv_amd.gesv_mixed(M, rhs, tol=1e-12, max_iter=50, check=False)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t1
+                fl = 2.0 / 3.0 * n ** 3 + 2.0 * n * n * nr
+                grid[f"kappa{kappa:.0e}_nrhs{nr}"] = {
+                    "ms": round(dt * 1e3, 3), "tflops": round(fl / dt / 1e12, 3),
+                    "iterations": inf["iterations"], "converged": inf["converged"],
+                    "backward_error": backward_error_gpu(M, Xg, rhs, torch)}
+                del Xg
+        # the FP64 LU solve with 64 right-hand sides, for the ratio at nrhs = 64
+        LU = A.clone()
+        x64 = b64.clone()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        piv, _ = tmv_amd.getrf(LU)
+        tmv_amd.getrs(LU, piv, x64)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        grid["fp64_lu_solve_nrhs64"] = {"ms": round(dt * 1e3, 3),
+                                        "backward_error": backward_error_gpu(A, x64, b64, torch)}
+        del LU, x64, b64, mats
+    A2 = None
+    torch.cuda.empty_cache()
 
     # ---- plain FP64 GPU LU solve on the same A, b ---------------------------
     t64 = []
@@ -487,6 +530,7 @@ def main():
                           "backward_error": be64},
         "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
         "mixed_kappa1e2": lowp or None,
+        "mixed_grid": grid,
         "dgemm_8192": gemm,
         "zgemm": zgemm,
         "batched_20000": batched,

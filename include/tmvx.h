@@ -154,6 +154,20 @@ int tmvx_getrs_c(int trans, int64_t n, int64_t nrhs, const float* LU, int64_t ld
                  const int64_t* piv, float* B, int64_t ldb, void* stream);
 
 /*
+ * From the factors of tmvx_getrf_z: log|det A| (FP64) and the complex sign
+ * det(P) * prod U_ii / |U_ii| (host outputs; synchronises), a zero diagonal
+ * giving logdet = -inf and sign 0 -- LUDiv::logDet for complex T
+ * (src/TMV_LUD.cpp:145-156; DiagMatrix::logDet, src/TMV_DiagMatrix.cpp:
+ * 59-90); and X := A^-1 (n x n, not aliasing LU; n <= 65535) as the solve
+ * of A X = I, the matrix TMV's LU_Inverse returns (src/TMV_LUInverse.cpp:
+ * 235-275).  A zero U diagonal makes tmvx_getri_z report TMVX_ERR_SINGULAR.
+ */
+int tmvx_logdet_z(int64_t n, const double* LU, int64_t lda, const int64_t* piv,
+                  double* logdet, double* sign_re, double* sign_im, void* stream);
+int tmvx_getri_z(int64_t n, const double* LU, int64_t lda, const int64_t* piv, double* X,
+                 int64_t ldx, void* stream);
+
+/*
  * QR decomposition by Householder reflections, in place: A (m x n, m >= n,
  * column-major) gets R on and above the diagonal and the reflectors below
  * it (first elements 1, implicit); beta (device, n) the reflectors' scales,

@@ -211,3 +211,32 @@ def test_cgetrf_complex64(orc, dev):
     X = dB.cpu().numpy().astype(np.complex128)
     r = np.abs(B - A32.astype(np.complex128) @ X).max() / (np.abs(A32).max() * np.abs(X).max() * n)
     assert r <= 8 * EPS32
+
+
+@pytest.mark.parametrize("n", [1, 3, 37, 300])
+def test_zlogdet_and_inverse(orc, dev, n):
+    """Complex log|det| and sign against numpy's slogdet (the determinant
+    does not depend on the pivot order), the inverse against A X = I and
+    the oracle's solve of A X = I on the GPU's factors."""
+    A = cplx(SEED + 800 + n, n, n)
+    dA, LU, P, info = gpu_zgetrf(A)
+    ld, sg = tmv_amd.logdet_complex(dA, torch.from_numpy(P).cuda())
+    nsg, nld = np.linalg.slogdet(A)
+    assert abs(ld - nld) <= 1e-10 * max(1.0, abs(nld))
+    assert abs(sg - nsg) <= 1e-10
+    X = tmv_amd.getri_complex(dA, torch.from_numpy(P).cuda()).cpu().numpy()
+    r = np.abs(A @ X - np.eye(n)).max() / (n * np.abs(A).max() * np.abs(X).max())
+    assert r <= 8 * EPS64
+    Xo = orc.getrs_complex(LU, P, np.eye(n, dtype=np.complex128))
+    assert np.abs(X - Xo).max() <= 1e-10 * max(1.0, np.abs(Xo).max())
+
+
+def test_zlogdet_singular(dev):
+    A = cplx(SEED + 901, 20, 20)
+    A[:, 3] = 0.0
+    dA, _, P, info = gpu_zgetrf(A)
+    assert info == 4
+    ld, sg = tmv_amd.logdet_complex(dA, torch.from_numpy(P).cuda())
+    assert ld == -np.inf and sg == 0
+    with pytest.raises(tmv_amd.SingularError):
+        tmv_amd.getri_complex(dA, torch.from_numpy(P).cuda())

@@ -57,6 +57,8 @@ _SIGS = {
     "tmvx_geqrf_f32": (_int, [_i64, _i64, _vp, _i64, _vp, _vp]),
     "tmvx_geqrs_f64": (_int, [_i64, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tmvx_geqrs_f32": (_int, [_i64, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "tmvx_logdet_z": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
+    "tmvx_getri_z": (_int, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tmvx_getri_f64": (_int, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tmvx_getri_f32": (_int, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "tmvx_logdet_f64": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
@@ -323,6 +325,29 @@ def geqrs(QR, beta, B, stream=None):
     _check(fn(m, n, Bm.shape[1], QR.data_ptr(), lda, beta.data_ptr(), Bm.data_ptr(), ldb,
               _stream(stream)), "geqrs")
     return B
+
+
+def logdet_complex(LU, piv, stream=None):
+    """(log|det A|, complex sign) from complex128 factors (TMV LUDiv::logDet
+    for complex T); sign 0 and logdet -inf for a singular matrix."""
+    n = LU.shape[0]
+    lda = _colmajor_ld(LU, "LU")
+    ld, sr, si = _d(0.0), _d(0.0), _d(0.0)
+    _check(lib().tmvx_logdet_z(n, LU.data_ptr(), lda, piv.data_ptr(), ctypes.byref(ld),
+                               ctypes.byref(sr), ctypes.byref(si), _stream(stream)),
+           "logdet_complex")
+    return ld.value, complex(sr.value, si.value)
+
+
+def getri_complex(LU, piv, stream=None):
+    """A^-1 from complex128 factors; returns a new column-major tensor."""
+    torch = _torch()
+    n = LU.shape[0]
+    lda = _colmajor_ld(LU, "LU")
+    X = colmajor(torch.empty(n, n, dtype=LU.dtype, device=LU.device))
+    _check(lib().tmvx_getri_z(n, LU.data_ptr(), lda, piv.data_ptr(), X.data_ptr(), max(n, 1),
+                              _stream(stream)), "getri_complex")
+    return X
 
 
 def getrf_tournament(A, stream=None):

@@ -338,3 +338,123 @@ int tmvx_getrs_c(int trans, int64_t n, int64_t nrhs, const float* LU, int64_t ld
 { return zgetrs_impl<float>(trans, n, nrhs, LU, lda, piv, B, ldb, (hipStream_t)stream); }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// Determinant and inverse from the complex factors.
+// ---------------------------------------------------------------------------
+namespace tmvx {
+
+// out[0] = sum log|U_ii| (FP64), out[1..2] = det(P) * prod U_ii / |U_ii|
+// (0 with out[0] = -inf when a diagonal is 0): LUDiv::logDet for complex T
+// (src/TMV_LUD.cpp:145-156; DiagMatrix::logDet multiplies the signs z/|z|,
+// src/TMV_DiagMatrix.cpp:59-90).  One workgroup of 256 threads.
+template <typename R>
+__global__ void __launch_bounds__(256)
+zlogdet_kernel(i64 n, const typename CplxT<R>::type* __restrict__ LU, i64 lda,
+               const i64* __restrict__ piv, double* __restrict__ out)
+{
+    __shared__ double s_l[256], s_re[256], s_im[256];
+    __shared__ int s_neg[256], s_zero[256];
+    const int t = threadIdx.x;
+    double sum = 0.0, pr = 1.0, pi = 0.0;
+    int neg = 0, zero = 0;
+    for (i64 i = t; i < n; i += blockDim.x) {
+        const double x = (double)LU[i + i * lda].x, y = (double)LU[i + i * lda].y;
+        const double a = hypot(x, y);
+        if (a == 0.0) zero = 1;
+        else {
+            sum += log(a);
+            const double ur = x / a, ui = y / a;
+            const double nr = pr * ur - pi * ui;
+            pi = pr * ui + pi * ur;
+            pr = nr;
+        }
+        if (piv[i] != i) neg ^= 1;
+    }
+    s_l[t] = sum; s_re[t] = pr; s_im[t] = pi; s_neg[t] = neg; s_zero[t] = zero;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) {
+            s_l[t] += s_l[t + off];
+            const double ar = s_re[t], ai = s_im[t], br = s_re[t + off], bi = s_im[t + off];
+            s_re[t] = ar * br - ai * bi;
+            s_im[t] = ar * bi + ai * br;
+            s_neg[t] ^= s_neg[t + off];
+            s_zero[t] |= s_zero[t + off];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double sg = s_neg[0] ? -1.0 : 1.0;
+        out[0] = s_zero[0] ? -INFINITY : s_l[0];
+        out[1] = s_zero[0] ? 0.0 : sg * s_re[0];
+        out[2] = s_zero[0] ? 0.0 : sg * s_im[0];
+    }
+}
+
+template <typename R>
+__global__ void zidentity_kernel(i64 n, typename CplxT<R>::type* __restrict__ X, i64 ldx)
+{
+    const i64 j = blockIdx.y;
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        typename CplxT<R>::type z;
+        z.x = i == j ? R(1) : R(0);
+        z.y = R(0);
+        X[i + j * ldx] = z;
+    }
+}
+
+template <typename R>
+int zlogdet_impl(i64 n, const R* LUr, i64 lda, const i64* piv, double* logdet, double* sre,
+                 double* sim, hipStream_t s)
+{
+    typedef typename CplxT<R>::type Z;
+    if (n < 0 || lda < (n > 0 ? n : 1) || !logdet || !sre || !sim) return TMVX_ERR_ARG;
+    if (n == 0) { *logdet = 0.0; *sre = 1.0; *sim = 0.0; return TMVX_OK; }
+    if (!LUr || !piv) return TMVX_ERR_ARG;
+    if (reinterpret_cast<uintptr_t>(LUr) % sizeof(Z)) return TMVX_ERR_ARG;
+    double* d = nullptr;
+    if (hipMalloc((void**)&d, 3 * sizeof(double)) != hipSuccess) return TMVX_ERR_ALLOC;
+    hipLaunchKernelGGL(zlogdet_kernel<R>, dim3(1), dim3(256), 0, s, n,
+                       reinterpret_cast<const Z*>(LUr), lda, piv, d);
+    int rc = hip_status(hipGetLastError());
+    double h[3] = {0.0, 0.0, 0.0};
+    if (rc == TMVX_OK && hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = TMVX_ERR_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess && rc == TMVX_OK) rc = TMVX_ERR_HIP;
+    hipFree(d);
+    *logdet = h[0];
+    *sre = h[1];
+    *sim = h[2];
+    return rc;
+}
+
+template <typename R>
+int zgetri_impl(i64 n, const R* LU, i64 lda, const i64* piv, R* Xr, i64 ldx, hipStream_t s)
+{
+    typedef typename CplxT<R>::type Z;
+    if (n < 0 || lda < (n > 0 ? n : 1) || ldx < (n > 0 ? n : 1)) return TMVX_ERR_ARG;
+    if (n == 0) return TMVX_OK;
+    if (!LU || !piv || !Xr) return TMVX_ERR_ARG;
+    if (n > 65535) return TMVX_ERR_UNSUPPORTED;
+    if (reinterpret_cast<uintptr_t>(Xr) % sizeof(Z)) return TMVX_ERR_ARG;
+    const i64 gx = (n + 255) / 256;
+    hipLaunchKernelGGL(zidentity_kernel<R>, dim3((unsigned)(gx < 64 ? gx : 64), (unsigned)n),
+                       dim3(256), 0, s, n, reinterpret_cast<Z*>(Xr), ldx);
+    TMVX_CHECK_LAUNCH();
+    return zgetrs_impl<R>(0, n, n, LU, lda, piv, Xr, ldx, s);
+}
+
+} // namespace tmvx
+
+extern "C" {
+
+int tmvx_logdet_z(int64_t n, const double* LU, int64_t lda, const int64_t* piv, double* logdet,
+                  double* sign_re, double* sign_im, void* stream)
+{ return tmvx::zlogdet_impl<double>(n, LU, lda, piv, logdet, sign_re, sign_im, (hipStream_t)stream); }
+
+int tmvx_getri_z(int64_t n, const double* LU, int64_t lda, const int64_t* piv, double* X,
+                 int64_t ldx, void* stream)
+{ return tmvx::zgetri_impl<double>(n, LU, lda, piv, X, ldx, (hipStream_t)stream); }
+
+} // extern "C"

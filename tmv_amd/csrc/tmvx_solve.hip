@@ -5,104 +5,81 @@
 // sweeps src/TMV_TriDiv_V.cpp:147-218, 321-383) and permuteRows /
 // reversePermuteRows (src/TMV_Matrix.cpp:865-930).
 //
-// op(A) X = B is solved by blocked substitution over diagonal blocks of
-// TRSM_NB rows, one launch per block.  W holds the running right-hand side
-// (a copy of B, optionally gathered through a row permutation); the step
-// kernel for diagonal block [b0, b1):
-//   - every workgroup loads the diagonal block of op(A) and W[b0:b1, :]
-//     into LDS and solves it by substitution (one wave per RHS column,
-//     x_k broadcast by shuffle; like TMV it skips zero entries and flags a
-//     zero diagonal met by a non-zero entry);
-//   - workgroup 0 writes X_b to B (optionally scattered through a row
+// op(A) X = B with many right-hand sides is solved by blocked substitution
+// over diagonal blocks of TRSM2_NB = 128 rows.  W holds the running
+// right-hand side (a copy of B, optionally gathered through a row
+// permutation); for diagonal block [b0, b1):
+//   - trsm_diag_kernel stages the block of op(A) in LDS and solves it for
+//     every column (one wave per column, x_k broadcast by shuffle; like TMV
+//     it skips zero entries and flags a zero diagonal met by a non-zero
+//     entry), writing X_b to W and to B (optionally scattered through a row
 //     permutation);
-//   - each workgroup updates its slab of the remaining rows,
-//     W[i, :] -= op(A)[i, b0:b1] * X_b.
+//   - the rows still to come take the block's contribution on the MFMA
+//     GEMM: W[rest, :] -= op(A)[rest, b0:b1] X_b.
 #include "tmvx_lu.h"
 
 namespace tmvx {
 
-constexpr int TRSM_NB = 32;
-constexpr int TRSM_ROWS = 256;   // trailing rows per workgroup (one per thread)
+// Diagonal block of the blocked solve: op(A)[b0:b0+bs, b0:b0+bs] X = W[b0:b0+bs, :]
+// for every right-hand side; bs <= TRSM2_NB.  The block is staged in LDS
+// once per workgroup; one wave per right-hand side column, lane l holding
+// rows l and l + 64, column sweep with the solved entry broadcast by a
+// shuffle (zero entries skipped, a zero diagonal met by a non-zero entry
+// flagged).  X goes back to W and to B (optionally scattered through a row
+// permutation).
+constexpr int TRSM2_NB = 128;
 
-template <typename T, int NB>
+template <typename T>
 __global__ void __launch_bounds__(256)
-trsm_step_kernel(int lower, int trans, int unit, i64 n, i64 nrhs,
-                 const T* __restrict__ A, i64 lda, T* __restrict__ W, i64 ldw,
-                 T* __restrict__ B, i64 ldb, const i64* __restrict__ out_perm,
-                 i64 b0, int bs, int* singular)
+trsm_diag_kernel(int lower, int trans, int unit, i64 nrhs, const T* __restrict__ A, i64 lda,
+                 T* __restrict__ W, i64 ldw, T* __restrict__ B, i64 ldb,
+                 const i64* __restrict__ out_perm, i64 b0, int bs, int* singular)
 {
-    __shared__ T D[NB][NB + 1];
-    __shared__ T X[NB][NB + 1];       // X[i][jj] for a chunk of NB RHS columns
+    __shared__ T D[TRSM2_NB][TRSM2_NB + 1];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    auto opA = [&](i64 i, i64 j) -> T {
-        return trans ? A[j + i * lda] : A[i + j * lda];
-    };
-    for (int e = t; e < NB * NB; e += blockDim.x) {
-        const int i = e % NB, j = e / NB;
-        D[i][j] = (i < bs && j < bs) ? opA(b0 + i, b0 + j) : T(0);
+    for (int e = t; e < TRSM2_NB * TRSM2_NB; e += blockDim.x) {
+        const int i = e % TRSM2_NB, j = e / TRSM2_NB;
+        T v = T(0);
+        if (i < bs && j < bs) v = trans ? A[(b0 + j) + (b0 + i) * lda] : A[(b0 + i) + (b0 + j) * lda];
+        D[i][j] = v;
     }
-    const i64 trail_lo = lower ? b0 + bs : 0;
-    const i64 trail_hi = lower ? n : b0;
-    const i64 row = trail_lo + (i64)blockIdx.x * TRSM_ROWS + t;
-    const bool has_row = row < trail_hi;
-    T arow[NB];
-#pragma unroll
-    for (int k = 0; k < NB; ++k)
-        arow[k] = (has_row && k < bs) ? opA(row, b0 + k) : T(0);
-
-    for (i64 j0 = 0; j0 < nrhs; j0 += NB) {
-        const int jn = (int)((nrhs - j0) < NB ? (nrhs - j0) : NB);
-        __syncthreads();
-        for (int e = t; e < NB * NB; e += blockDim.x) {
-            const int i = e % NB, jj = e / NB;
-            X[i][jj] = (i < bs && jj < jn) ? W[(b0 + i) + (j0 + jj) * ldw] : T(0);
-        }
-        __syncthreads();
-        for (int jj = wave; jj < jn; jj += 4) {
-            T x = lane < bs ? X[lane][jj] : T(0);
+    __syncthreads();
+    const i64 col = (i64)blockIdx.x * 4 + wave;
+    if (col >= nrhs) return;                 // no barrier below
+    T* w = W + col * ldw + b0;
+    T x0 = lane < bs ? w[lane] : T(0);
+    T x1 = lane + 64 < bs ? w[lane + 64] : T(0);
+    for (int q = 0; q < bs; ++q) {
+        const int k = lower ? q : bs - 1 - q;
+        T xk = __shfl(k < 64 ? x0 : x1, k & 63, 64);
+        if (xk != T(0)) {
+            if (!unit) {
+                const T d = D[k][k];
+                if (d == T(0)) {
+                    if (lane == 0) atomicOr(singular, 1);
+                } else {
+                    xk = xk / d;
+                }
+                if (lane == (k & 63)) { if (k < 64) x0 = xk; else x1 = xk; }
+            }
             if (lower) {
-                for (int k = 0; k < bs; ++k) {
-                    T xk = __shfl(x, k, 64);
-                    if (!unit && xk != T(0)) {
-                        const T d = D[k][k];
-                        if (d == T(0)) { if (lane == 0) atomicOr(singular, 1); }
-                        else xk = xk / d;
-                        if (lane == k) x = xk;
-                    }
-                    if (lane > k && lane < bs) x -= D[lane][k] * xk;
-                }
+                if (lane > k && lane < bs) x0 -= D[lane][k] * xk;
+                if (lane + 64 > k && lane + 64 < bs) x1 -= D[lane + 64][k] * xk;
             } else {
-                for (int k = bs - 1; k >= 0; --k) {
-                    T xk = __shfl(x, k, 64);
-                    if (!unit && xk != T(0)) {
-                        const T d = D[k][k];
-                        if (d == T(0)) { if (lane == 0) atomicOr(singular, 1); }
-                        else xk = xk / d;
-                        if (lane == k) x = xk;
-                    }
-                    if (lane < k) x -= D[lane][k] * xk;
-                }
-            }
-            if (lane < bs) X[lane][jj] = x;
-        }
-        __syncthreads();
-        if (blockIdx.x == 0) {
-            for (int e = t; e < NB * NB; e += blockDim.x) {
-                const int i = e % NB, jj = e / NB;
-                if (i < bs && jj < jn) {
-                    const i64 orow = out_perm ? out_perm[b0 + i] : b0 + i;
-                    B[orow + (j0 + jj) * ldb] = X[i][jj];
-                }
+                if (lane < k) x0 -= D[lane][k] * xk;
+                if (lane + 64 < k) x1 -= D[lane + 64][k] * xk;
             }
         }
-        if (has_row) {
-            for (int jj = 0; jj < jn; ++jj) {
-                T s = W[row + (j0 + jj) * ldw];
-#pragma unroll
-                for (int k = 0; k < NB; ++k) s -= arow[k] * X[k][jj];
-                W[row + (j0 + jj) * ldw] = s;
-            }
-        }
+    }
+    if (lane < bs) {
+        w[lane] = x0;
+        const i64 orow = out_perm ? out_perm[b0 + lane] : b0 + lane;
+        B[orow + col * ldb] = x0;
+    }
+    if (lane + 64 < bs) {
+        w[lane + 64] = x1;
+        const i64 orow = out_perm ? out_perm[b0 + lane + 64] : b0 + lane + 64;
+        B[orow + col * ldb] = x1;
     }
 }
 
@@ -136,25 +113,38 @@ int trsm_impl(int upper, int trans, int unit, i64 n, i64 nrhs, const T* A, i64 l
     hipLaunchKernelGGL(gather_rows_kernel<T>, dim3((unsigned)((n + 255) / 256), (unsigned)nrhs),
                        dim3(256), 0, s, n, nrhs, B, ldb, in_perm, W, n);
     int rc = hip_status(hipGetLastError());
-    constexpr int NB = TRSM_NB;
+    // Blocks of TRSM2_NB rows: the diagonal block by trsm_diag_kernel, then
+    // the rows still to come take the block's contribution in one MFMA GEMM,
+    // W[rest] -= op(A)[rest, blk] X_blk.
+    constexpr int NB = TRSM2_NB;
     const i64 nblk = (n + NB - 1) / NB;
+    const unsigned gd = (unsigned)((nrhs + 3) / 4);
     for (i64 q = 0; q < nblk && rc == TMVX_OK; ++q) {
         const i64 blk = lower ? q : nblk - 1 - q;
         const i64 b0 = blk * NB;
         const int bs = (int)((n - b0) < NB ? (n - b0) : NB);
-        const i64 ntrail = lower ? n - b0 - bs : b0;
-        const unsigned grid = (unsigned)(ntrail > 0 ? (ntrail + TRSM_ROWS - 1) / TRSM_ROWS : 1);
-        hipLaunchKernelGGL((trsm_step_kernel<T, NB>), dim3(grid), dim3(256), 0, s, lower,
-                           trans, unit, n, nrhs, A, lda, W, n, B, ldb, out_perm, b0, bs,
-                           d_singular);
-        rc = hip_status(hipGetLastError());
+        hipLaunchKernelGGL((trsm_diag_kernel<T>), dim3(gd), dim3(256), 0, s, lower, trans, unit,
+                           nrhs, A, lda, W, n, B, ldb, out_perm, b0, bs, d_singular);
+        if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
+        const i64 r0 = lower ? b0 + bs : 0;          // rows still to come: [r0, r1)
+        const i64 r1 = lower ? n : b0;
+        if (r1 <= r0) continue;
+        // op(A)[r0:r1, b0:b0+bs]: column-major block of A, or (trans) the
+        // transpose of A[b0:b0+bs, r0:r1] read row-major
+        const T* Ab = trans ? A + b0 + r0 * lda : A + r0 + b0 * lda;
+        rc = gemm_launch<T>(trans ? 1 : 0, 0, r1 - r0, nrhs, bs, T(-1), Ab, lda, W + b0, n, T(1),
+                            W + r0, n, s);
     }
     hipFreeAsync(W, s);
     return rc;
 }
 
-// Right-hand sides up to this count use the single-launch solve.
-constexpr i64 TRSV_MAX_RHS = 16;
+// Right-hand sides up to this count use the single-launch solve (a build
+// option for timing the crossover: -DTMVX_TRSV_MAX_RHS=k).
+#ifndef TMVX_TRSV_MAX_RHS
+#define TMVX_TRSV_MAX_RHS 16
+#endif
+constexpr i64 TRSV_MAX_RHS = TMVX_TRSV_MAX_RHS;
 
 template <typename T>
 int getrs_perm(int trans, i64 n, i64 nrhs, const T* LU, i64 lda, const i64* perm, T* B,
