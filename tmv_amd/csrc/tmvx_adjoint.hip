@@ -17,6 +17,7 @@
 // read unless accumulating.  The recompute variant factors copies of the
 // original matrices first, for callers that did not keep the factors.
 #include "tmvx_lu.h"
+#include "tmvx_pivot.h"
 #include "tmvx_wave_solve.h"
 #include "../../include/tmvx_batched.h"
 
@@ -125,6 +126,82 @@ static int adjoint_launch(int64_t count, const int32_t* n, const int32_t* nrhs, 
 
 // Factor every A[i] in place: systems with n <= TMVX_BATCH_NMAX through one
 // batched launch, larger ones through the blocked LU, one after another.
+// Medium systems (above the register-resident batched kernel's limit, up to
+// WG_NMAX): one 1024-thread workgroup factors one system in place, column
+// by column, in global memory (L2-resident at these sizes): TMV's pivot
+// rule (largest |a|, first row on ties), its underflow rule (the column is
+// zeroed from the diagonal down and interchanged with itself), the column
+// scaled by the pivot's reciprocal, then the rank-1 update of the trailing
+// matrix spread over all threads.  info[sys] = 1-based first zero pivot.
+constexpr int WG_NMAX = 1024;
+
+template <typename T>
+__global__ void __launch_bounds__(1024)
+lu_wg_kernel(const int* __restrict__ order, const int32_t* __restrict__ nv, T* const* A,
+             int64_t* const* piv, int32_t* __restrict__ info)
+{
+    constexpr int NW = 1024 / 64;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    __shared__ T s_v[NW];
+    __shared__ int s_i[NW];
+    __shared__ int s_p, s_zero;
+    __shared__ T s_inv;
+    const int sys = order[blockIdx.x];
+    const int n = nv[sys];
+    T* a = A[sys];
+    int64_t* pv = piv[sys];
+    int first_zero = 0;
+    for (int j = 0; j < n; ++j) {
+        T bv = T(-1);
+        int bi = INT_MAX;
+        for (int i = j + t; i < n; i += 1024) {
+            const T v = fabs(a[i + (i64)j * n]);
+            if (v > bv) { bv = v; bi = i; }
+        }
+        wave_argmax(bv, bi);
+        if (lane == 0) { s_v[wave] = bv; s_i[wave] = bi; }
+        __syncthreads();
+        if (t == 0) {
+            T v = s_v[0];
+            int ii = s_i[0];
+            for (int q = 1; q < NW; ++q)
+                if (better(s_v[q], s_i[q], v, ii)) { v = s_v[q]; ii = s_i[q]; }
+            const bool zero = !(v >= tiny<T>());
+            s_zero = zero ? 1 : 0;
+            s_p = zero ? j : ii;
+            pv[j] = zero ? j : ii;
+            if (zero && first_zero == 0) first_zero = j + 1;
+        }
+        __syncthreads();
+        if (s_zero) {
+            for (int i = j + t; i < n; i += 1024) a[i + (i64)j * n] = T(0);
+            __syncthreads();
+            continue;
+        }
+        const int p = s_p;
+        if (p != j)
+            for (int c = t; c < n; c += 1024) {
+                const T x = a[j + (i64)c * n];
+                a[j + (i64)c * n] = a[p + (i64)c * n];
+                a[p + (i64)c * n] = x;
+            }
+        __syncthreads();
+        if (t == 0) s_inv = T(1) / a[j + (i64)j * n];
+        __syncthreads();
+        const T inv = s_inv;
+        for (int i = j + 1 + t; i < n; i += 1024) a[i + (i64)j * n] *= inv;
+        __syncthreads();
+        const int m = n - j - 1;
+        const i64 tot = (i64)m * m;
+        for (i64 e = t; e < tot; e += 1024) {
+            const int i = j + 1 + (int)(e % m), c = j + 1 + (int)(e / m);
+            a[i + (i64)c * n] -= a[i + (i64)j * n] * a[j + (i64)c * n];
+        }
+        __syncthreads();
+    }
+    if (t == 0) info[sys] = first_zero;
+}
+
 template <typename T>
 static int getrf_batched_impl(int64_t count, const int32_t* n_host, T* const* A, int64_t* const* piv,
                               int32_t* info, hipStream_t s)
@@ -139,10 +216,33 @@ static int getrf_batched_impl(int64_t count, const int32_t* n_host, T* const* A,
     TMVX_HIP(hipMemcpyAsync(hA.data(), A, sizeof(T*) * count, hipMemcpyDeviceToHost, s));
     TMVX_HIP(hipMemcpyAsync(hP.data(), piv, sizeof(int64_t*) * count, hipMemcpyDeviceToHost, s));
     TMVX_HIP(hipStreamSynchronize(s));
-    std::vector<int> small, big;
+    std::vector<int> small, medium, big;
     for (int64_t i = 0; i < count; ++i)
-        (n_host[i] <= TMVX_BATCH_NMAX ? small : big).push_back((int)i);
+        (n_host[i] <= TMVX_BATCH_NMAX ? small : n_host[i] <= WG_NMAX ? medium : big)
+            .push_back((int)i);
     int rc = TMVX_OK;
+    if (!medium.empty()) {
+        int* dord = nullptr;
+        int32_t* dn = nullptr;
+        if (hipMalloc((void**)&dord, sizeof(int) * medium.size()) != hipSuccess ||
+            hipMalloc((void**)&dn, sizeof(int32_t) * (size_t)count) != hipSuccess)
+            rc = TMVX_ERR_ALLOC;
+        if (rc == TMVX_OK &&
+            (hipMemcpyAsync(dord, medium.data(), sizeof(int) * medium.size(),
+                            hipMemcpyHostToDevice, s) != hipSuccess ||
+             hipMemcpyAsync(dn, n_host, sizeof(int32_t) * (size_t)count, hipMemcpyHostToDevice,
+                            s) != hipSuccess))
+            rc = TMVX_ERR_HIP;
+        if (rc == TMVX_OK) {
+            hipLaunchKernelGGL(lu_wg_kernel<T>, dim3((unsigned)medium.size()), dim3(1024), 0, s,
+                               (const int*)dord, (const int32_t*)dn, A, piv, info);
+            rc = hip_status(hipGetLastError());
+        }
+        if (hipStreamSynchronize(s) != hipSuccess && rc == TMVX_OK) rc = TMVX_ERR_HIP;
+        hipFree(dord);
+        hipFree(dn);
+        if (rc != TMVX_OK) return rc;
+    }
     if (!small.empty()) {
         const size_t ns = small.size();
         std::vector<int32_t> sn(ns), snr(ns, 0);
@@ -186,9 +286,9 @@ static int getrf_batched_impl(int64_t count, const int32_t* n_host, T* const* A,
         hipFree(dptr);
         hipFree(dinfo);
     }
-    // Systems above the batched kernel's limit: the blocked LU, one system
-    // per call, from up to 16 host threads with a stream each (every call
-    // synchronises its own stream, so the calls overlap on the GPU).
+    // Systems above WG_NMAX: the blocked LU, one system per call, from up to
+    // 16 host threads with a stream each (every call synchronises its own
+    // stream, so the calls overlap on the GPU).
     if (rc == TMVX_OK && !big.empty()) {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess) return TMVX_ERR_HIP;
