@@ -8,11 +8,17 @@
 // Workgroups take row blocks in ticket order (an atomic counter), so a
 // workgroup only ever waits for blocks held by workgroups that started
 // before it -- the schedule cannot deadlock.  Block b (in solve order)
-// accumulates acc = sum_c op(A)[b, c] x_c over finished blocks c as their
-// ready flags appear, then solves its diagonal block with one lane per row
-// (row of the diagonal block in registers, x_k broadcast by readlane),
-// publishes x_b with device-coherent stores and raises its flag.  Up to
+// accumulates acc = sum_c op(A)[b, c] x_c over finished blocks c, then
+// solves its diagonal block with one lane per row (row of the diagonal
+// block in registers, x_k broadcast by readlane) and publishes x_b.  Up to
 // SV_RHS right-hand sides per launch, one wave each.  Every wait is bounded.
+//
+// x_b is published as tagged words: each 64-bit word carries the launch's
+// epoch in its upper half and 32 bits of x in its lower half (two words per
+// double), written and read with device-coherent 64-bit accesses.  A
+// consumer polls the words themselves, so a block's result is one one-way
+// trip away from the next block, with no ready flag and no cache
+// write-back or invalidation on the chain.
 #include "tmvx_lu.h"
 
 namespace tmvx {
@@ -21,19 +27,57 @@ constexpr int SV_NB = 64;
 constexpr int SV_RHS = 4;          // RHS per launch (one wave per RHS)
 constexpr unsigned SV_SPIN_LIMIT = 1u << 22;
 
-template <typename T>
-__device__ __forceinline__ void sv_cstore(T* p, T v)
+typedef unsigned long long u64;
+
+template <typename T> struct SvWords { static constexpr int n = (int)(sizeof(T) / 4); };
+
+__device__ __forceinline__ void sv_cstore(u64* p, u64 v)
 { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-template <typename T>
-__device__ __forceinline__ T sv_cload(const T* p)
-{ return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ u64 sv_cload(const u64* p)
+{ return __hip_atomic_load(const_cast<u64*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+__device__ __forceinline__ void sv_publish(u64* p, float x, unsigned epoch)
+{ sv_cstore(p, ((u64)epoch << 32) | (u64)__float_as_uint(x)); }
+__device__ __forceinline__ void sv_publish(u64* p, double x, unsigned epoch)
+{
+    const u64 b = (u64)__double_as_longlong(x);
+    sv_cstore(p, ((u64)epoch << 32) | (b & 0xffffffffull));
+    sv_cstore(p + 1, ((u64)epoch << 32) | (b >> 32));
+}
+// true (and x set) once every word of the value carries `epoch`
+__device__ __forceinline__ bool sv_fetch(const u64* p, float& x, unsigned epoch)
+{
+    const u64 w = sv_cload(p);
+    if ((unsigned)(w >> 32) != epoch) return false;
+    x = __uint_as_float((unsigned)w);
+    return true;
+}
+__device__ __forceinline__ bool sv_fetch(const u64* p, double& x, unsigned epoch)
+{
+    const u64 lo = sv_cload(p), hi = sv_cload(p + 1);
+    if ((unsigned)(lo >> 32) != epoch || (unsigned)(hi >> 32) != epoch) return false;
+    x = __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+    return true;
+}
+
+// -DTMVX_TRSV_TRACE: wall-clock marks per block (solve order q), 8 slots:
+// 0 walk reaches block q-1, 1 x_{q-1} seen, 2 partial sums added,
+// 3 diagonal block solved, 4 x_q published, 5 workgroup start.
+#ifdef TMVX_TRSV_TRACE
+__device__ unsigned long long* g_trsv_trace;
+#define TRSV_TRACE(qq, slot)                                                   \
+    do {                                                                       \
+        if (g_trsv_trace && lane == 0) g_trsv_trace[(qq) * 8 + (slot)] = wall_clock64(); \
+    } while (0)
+#else
+#define TRSV_TRACE(qq, slot) do {} while (0)
+#endif
 
 struct SvWork {
     unsigned* ticket;     // next block ticket
-    unsigned* flags;      // [nblk] = epoch when the block's x is published
     int* err;             // timeout
     int* singular;        // zero diagonal met a non-zero entry
-    unsigned epoch;       // value a finished block's flag takes
+    unsigned epoch;       // tag of this launch's published words
 };
 
 // Solve op(A) X = B for rows of B given through `in_perm` (gather), X
@@ -41,146 +85,174 @@ struct SvWork {
 // `lower`: op(A) lower triangular (forward order), else upper (backward).
 // op(A)(i, j) = trans ? A[j + i*lda] : A[i + j*lda].
 //
-// Wave w of the block's workgroup owns right-hand side w; lane i owns row
-// b0 + i.  A finished block's x arrives as one coherent load per lane and
-// is broadcast with readlane, so contributions need no barrier.  Ready
-// flags are checked in bulk: one poll reads the flags of the next 64
-// blocks in solve order and the longest finished prefix is consumed
-// without further polls (contributions stay in solve order, so results
-// are deterministic).  The diagonal block and the reciprocals of its
-// diagonal are loaded first, off the dependency chain.
+// The workgroup's 4 waves are shared out as wpr = 4 / nr waves per
+// right-hand side; lane i owns row b0 + i.  Sub-wave s of a right-hand
+// side walks the finished blocks qc = s, s + wpr, ... (each block of op(A)
+// loaded before the wait for the x it multiplies; x arrives as the lane's
+// tagged word(s) and is broadcast with readlane).  The partial sums are
+// added in sub-wave order through LDS, so results are deterministic, and
+// the leading sub-wave solves the diagonal block (loaded first of all, off
+// the dependency chain) and publishes x_b.
 template <typename T>
 __global__ void __launch_bounds__(256)
 trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
                        const T* __restrict__ A, i64 lda,
                        const T* __restrict__ Bin, i64 ldb, const i64* __restrict__ in_perm,
                        T* __restrict__ Xout, i64 ldx, const i64* __restrict__ out_perm,
-                       T* __restrict__ xpub, SvWork wk)
+                       u64* __restrict__ xpub, SvWork wk)
 {
-    constexpr int NB = SV_NB;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;   // wave = RHS column
+    constexpr int NB = SV_NB, W = SvWords<T>::n, NW = 4;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const i64 nblk = (n + NB - 1) / NB;
     __shared__ unsigned s_tk;
-    if (t == 0) s_tk = atomicAdd(wk.ticket, 1u);
+    __shared__ int s_fail;
+    __shared__ T s_part[NW][NB];
+    if (t == 0) { s_tk = atomicAdd(wk.ticket, 1u); s_fail = 0; }
     __syncthreads();
     const i64 q = s_tk;                       // position in solve order
-    if (q >= nblk) return;
+    if (q >= nblk) return;                    // (whole workgroup)
+    if (wave == 0) TRSV_TRACE(q, 5);
+    const int wpr = nr >= 1 && nr <= NW ? NW / nr : 1;   // waves per right-hand side
+    const int rhs = wave / wpr, sub = wave % wpr;
+    const bool active = rhs < nr;
     const i64 blk = lower ? q : nblk - 1 - q;
     const i64 b0 = blk * NB;
     const int bs = (int)((n - b0) < NB ? (n - b0) : NB);
     const bool row_ok = lane < bs;
-    auto opA = [&](i64 i, i64 j) -> T { return trans ? A[j + i * lda] : A[i + j * lda]; };
-
-    if (wave < nr) {
-        // diagonal block row and reciprocal of the diagonal, loaded first
-        T drow[NB];
-        static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
-            constexpr int k = decltype(kc)::value;
-            drow[k] = (row_ok && k < bs) ? opA(b0 + lane, b0 + k) : T(0);
-        });
-        T rdiag = T(1);
-        if (!unit && row_ok) {
-            const T d = opA(b0 + lane, b0 + lane);
-            rdiag = d == T(0) ? T(0) : T(1) / d;
-        }
-        T acc = T(0);
-        if (row_ok) {
-            i64 src = b0 + lane;
-            if (in_perm) { src = in_perm[b0 + lane]; if (src < 0 || src >= n) src = b0 + lane; }
-            acc = Bin[src + wave * ldb];
-        }
-        bool failed = false;
-        i64 ready = 0;                        // blocks [0, ready) in solve order are done
-        for (i64 qc = 0; qc < q && !failed; ++qc) {
-            const i64 cb = lower ? qc : nblk - 1 - qc;
-            const i64 c0 = cb * NB;
-            const int cs = (int)((n - c0) < NB ? (n - c0) : NB);
-            T ablk[NB];
+    // op(A)(b0 + lane, j) = arow[j * astr]
+    const T* arow = trans ? A + (b0 + lane) * lda : A + (b0 + lane);
+    const i64 astr = trans ? 1 : lda;
+    // op(A)(b0 + lane, c0 + k) for k < cs (0 beyond, and on rows past n)
+    auto load_block = [&](T (&blk_)[NB], i64 c0, int cs) __attribute__((always_inline)) {
+        const T* ap = arow + c0 * astr;
+        if (row_ok && cs == NB) {
             static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
                 constexpr int k = decltype(kc)::value;
-                ablk[k] = (row_ok && k < cs) ? opA(b0 + lane, c0 + k) : T(0);
-            });
-            if (qc >= ready) {
-                unsigned spins = 0;
-                while (true) {
-                    const i64 qq = qc + lane;
-                    bool done = true;
-                    if (qq < q) {
-                        const i64 fb = lower ? qq : nblk - 1 - qq;
-                        done = __hip_atomic_load(wk.flags + fb, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT) == wk.epoch;
-                    }
-                    const unsigned long long bal = __ballot(done);
-                    const int run = bal == ~0ull ? 64 : __builtin_ctzll(~bal);
-                    if (run > 0) { ready = qc + run; break; }
-                    if (++spins > SV_SPIN_LIMIT ||
-                        __hip_atomic_load(wk.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                        if (lane == 0) {
-                            __hip_atomic_store(wk.err, 1, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-                            atomicOr(wk.singular, 2);      // bit 1: a bounded wait expired
-                        }
-                        failed = true;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                if (failed) break;
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            }
-            const T xc = lane < cs ? sv_cload(xpub + c0 + lane + (i64)wave * n) : T(0);
-            T s0 = T(0), s1 = T(0);
-            static_for<0, NB / 2>([&](auto kc) __attribute__((always_inline)) {
-                constexpr int k = 2 * decltype(kc)::value;
-                s0 += ablk[k] * readlane(xc, k);
-                s1 += ablk[k + 1] * readlane(xc, k + 1);
-            });
-            acc -= s0 + s1;
-        }
-        // Diagonal block: lane i keeps row i of op(A)[b, b].
-        T x = acc;
-        bool sing = false;
-        if (lower) {
-            static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
-                constexpr int k = decltype(kc)::value;
-                if (k < bs) {
-                    if (!unit && lane == k && x != T(0)) {
-                        if (rdiag == T(0)) sing = true; else x = x * rdiag;
-                    }
-                    const T xk = readlane(x, k);
-                    if (lane > k) x -= drow[k] * xk;
-                }
+                blk_[k] = ap[k * astr];
             });
         } else {
             static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
-                constexpr int k = NB - 1 - decltype(kc)::value;
-                if (k < bs) {
-                    if (!unit && lane == k && x != T(0)) {
-                        if (rdiag == T(0)) sing = true; else x = x * rdiag;
-                    }
-                    const T xk = readlane(x, k);
-                    if (lane < k) x -= drow[k] * xk;
-                }
+                constexpr int k = decltype(kc)::value;
+                blk_[k] = (row_ok && k < cs) ? ap[k * astr] : T(0);
             });
         }
-        if (sing) atomicOr(wk.singular, 1);
-        if (row_ok && !failed) {
-            sv_cstore(xpub + b0 + lane + (i64)wave * n, x);
-            i64 dst = b0 + lane;
-            if (out_perm) { dst = out_perm[b0 + lane]; if (dst < 0 || dst >= n) dst = b0 + lane; }
-            Xout[dst + wave * ldx] = x;
-        }
+    };
+    const bool lead = active && sub == 0;
+
+    // diagonal block row and reciprocal of the diagonal, loaded first
+    T drow[NB];
+    if (lead) load_block(drow, b0, bs);
+    T rdiag = T(1);
+    if (lead && !unit && row_ok) {
+        const T d = arow[(b0 + lane) * astr];
+        rdiag = d == T(0) ? T(0) : T(1) / d;
     }
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (t == 0)
-        __hip_atomic_store(wk.flags + blk, wk.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    T acc = T(0);
+    if (lead && row_ok) {
+        i64 src = b0 + lane;
+        if (in_perm) { src = in_perm[b0 + lane]; if (src < 0 || src >= n) src = b0 + lane; }
+        acc = Bin[src + rhs * ldb];
+    }
+    u64* const xcol = xpub + (i64)(active ? rhs : 0) * n * W;
+    bool failed = false;
+    for (i64 qc = sub; active && qc < q && !failed; qc += wpr) {
+        const i64 cb = lower ? qc : nblk - 1 - qc;
+        const i64 c0 = cb * NB;
+        const int cs = (int)((n - c0) < NB ? (n - c0) : NB);
+        T ablk[NB];
+        load_block(ablk, c0, cs);
+        T xc = T(0);
+        unsigned spins = 0;
+        if (qc == q - 1) TRSV_TRACE(q, 0);
+        while (true) {
+            const bool ok = lane >= cs || sv_fetch(xcol + (c0 + lane) * W, xc, wk.epoch);
+            if (__ballot(!ok) == 0ull) break;
+            if (++spins > SV_SPIN_LIMIT ||
+                ((spins & 255) == 0 &&
+                 __hip_atomic_load(wk.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                if (lane == 0) {
+                    __hip_atomic_store(wk.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    atomicOr(wk.singular, 2);      // bit 1: a bounded wait expired
+                    s_fail = 1;
+                }
+                failed = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (failed) break;
+        if (qc == q - 1) TRSV_TRACE(q, 1);
+        T s0 = T(0), s1 = T(0);
+        static_for<0, NB / 2>([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = 2 * decltype(kc)::value;
+            s0 += ablk[k] * readlane(xc, k);
+            s1 += ablk[k + 1] * readlane(xc, k + 1);
+        });
+        acc -= s0 + s1;
+    }
+    if (wpr > 1) {                           // partial sums, in sub-wave order
+        if (active && sub > 0) s_part[wave][lane] = acc;
+        __syncthreads();
+        if (lead)
+            for (int u = 1; u < wpr; ++u) acc += s_part[wave + u][lane];
+    } else {
+        __syncthreads();
+    }
+    if (!lead || s_fail) return;
+    TRSV_TRACE(q, 2);
+    // Diagonal block: lane i keeps row i of op(A)[b, b].  Step k: x_k is
+    // scaled by the reciprocal of its diagonal (a zero x_k stays as it is;
+    // a zero diagonal under a non-zero x_k is singular) and broadcast, and
+    // the rows still to come subtract op(A)(i, k) x_k.  Written with selects
+    // rather than branches: the 64 steps are the dependency chain.
+    T x = acc;
+    bool sing = false;
+    auto dstep = [&](auto kc) __attribute__((always_inline)) {
+        constexpr int k = decltype(kc)::value;
+        const T xr = readlane(x, k);
+        const T rk = readlane(rdiag, k);      // 1 for a unit diagonal
+        sing |= (xr != T(0)) & (rk == T(0));
+        const T xk = xr != T(0) ? xr * rk : xr;
+        const T upd = x - drow[k] * xk;
+        const bool later = lower ? lane > k : lane < k;
+        x = lane == k ? xk : (later ? upd : x);
+    };
+    if (lower) {
+        if (bs == NB)
+            static_for<0, NB>([&](auto kc) __attribute__((always_inline)) { dstep(kc); });
+        else
+            static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
+                if (decltype(kc)::value < bs) dstep(kc);
+            });
+    } else {
+        // upper: k = NB-1 .. 0 (rows past bs are skipped)
+        if (bs == NB)
+            static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
+                dstep(std::integral_constant<int, NB - 1 - decltype(kc)::value>{});
+            });
+        else
+            static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
+                constexpr int k = NB - 1 - decltype(kc)::value;
+                if (k < bs) dstep(std::integral_constant<int, k>{});
+            });
+    }
+    if (sing) atomicOr(wk.singular, 1);
+    TRSV_TRACE(q, 3);
+    if (row_ok) {
+        sv_publish(xcol + (b0 + lane) * W, x, wk.epoch);
+        TRSV_TRACE(q, 4);
+        i64 dst = b0 + lane;
+        if (out_perm) { dst = out_perm[b0 + lane]; if (dst < 0 || dst >= n) dst = b0 + lane; }
+        Xout[dst + rhs * ldx] = x;
+    }
 }
 
-__global__ void sv_init_kernel(unsigned* ticket, unsigned* flags, i64 nblk, int* err)
+// Per launch: ticket and error flag reset; before the first launch of a
+// solve also the published words (a fresh workspace may hold any tags).
+__global__ void sv_init_kernel(unsigned* ticket, u64* words, i64 nwords, int* err)
 {
     const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nblk) flags[i] = 0u;
+    if (i < nwords) words[i] = 0ull;
     if (i == 0) { *ticket = 0u; *err = 0; }
 }
 
@@ -195,28 +267,24 @@ int trsv_multi(int upper, int trans, int unit, i64 n, i64 nrhs, const T* A, i64 
     if (n <= 0 || nrhs <= 0) return TMVX_OK;
     const int lower = (upper == 0) != (trans != 0);
     const i64 nblk = (n + SV_NB - 1) / SV_NB;
+    if (nblk > (i64)0xffffffffu) return TMVX_ERR_UNSUPPORTED;
     char* ws = nullptr;
-    const size_t szx = sizeof(T) * (size_t)n * SV_RHS;
-    const size_t szf = sizeof(unsigned) * (size_t)nblk;
-    if (hipMallocAsync((void**)&ws, szx + szf + 256, s) != hipSuccess) return TMVX_ERR_ALLOC;
-    T* xpub = reinterpret_cast<T*>(ws);
+    const i64 nwords = n * SV_RHS * SvWords<T>::n;
+    const size_t szx = sizeof(u64) * (size_t)nwords;
+    if (hipMallocAsync((void**)&ws, szx + 256, s) != hipSuccess) return TMVX_ERR_ALLOC;
+    u64* xpub = reinterpret_cast<u64*>(ws);
     SvWork wk;
-    wk.flags = reinterpret_cast<unsigned*>(ws + szx);
-    wk.ticket = reinterpret_cast<unsigned*>(ws + szx + szf);
-    wk.err = reinterpret_cast<int*>(ws + szx + szf + 64);
+    wk.ticket = reinterpret_cast<unsigned*>(ws + szx);
+    wk.err = reinterpret_cast<int*>(ws + szx + 64);
     wk.singular = d_singular;
     int rc = TMVX_OK;
     unsigned epoch = 0;
     for (i64 j0 = 0; j0 < nrhs && rc == TMVX_OK; j0 += SV_RHS) {
         const int nr = (int)((nrhs - j0) < SV_RHS ? (nrhs - j0) : SV_RHS);
         ++epoch;
-        if (epoch == 1) {
-            hipLaunchKernelGGL(sv_init_kernel, dim3((unsigned)((nblk + 255) / 256)), dim3(256),
-                               0, s, wk.ticket, wk.flags, nblk, wk.err);
-        } else {
-            hipLaunchKernelGGL(sv_init_kernel, dim3(1), dim3(1), 0, s, wk.ticket, wk.flags,
-                               (i64)0, wk.err);
-        }
+        const i64 nz = epoch == 1 ? nwords : 0;
+        hipLaunchKernelGGL(sv_init_kernel, dim3((unsigned)((nz + 255) / 256 > 0 ? (nz + 255) / 256 : 1)),
+                           dim3(256), 0, s, wk.ticket, xpub, nz, wk.err);
         if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
         wk.epoch = epoch;
         hipLaunchKernelGGL(trsv_persistent_kernel<T>, dim3((unsigned)nblk), dim3(256), 0, s,
@@ -235,3 +303,11 @@ template int trsv_multi<float>(int, int, int, i64, i64, const float*, i64, const
                                const i64*, float*, i64, const i64*, int*, hipStream_t);
 
 } // namespace tmvx
+
+#ifdef TMVX_TRSV_TRACE
+extern "C" int tmvx_trsv_trace_set(unsigned long long* buf)
+{
+    return hipMemcpyToSymbol(HIP_SYMBOL(tmvx::g_trsv_trace), &buf, sizeof(buf)) == hipSuccess
+               ? 0 : -3;
+}
+#endif
