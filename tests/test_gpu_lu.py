@@ -159,3 +159,35 @@ def test_getrf_8192_backward_error(dev):
     nA = np.abs(A0).sum(axis=1).max()
     be = np.abs(R).max(axis=0) / (nA * np.abs(X).max(axis=0) + np.abs(B0).max(axis=0))
     assert np.all(be <= 8 * n * EPS64), be.max()
+
+
+def _swapped(piv, b):
+    """Rows of b after the interchanges piv[0], piv[1], ... in order."""
+    b = b.copy()
+    for k, p in enumerate(piv):
+        if 0 <= p < len(b) and p != k:
+            b[[k, p]] = b[[p, k]]
+    return b
+
+
+@pytest.mark.parametrize("n,kind", [(1, "far"), (63, "far"), (5000, "far"), (5000, "near"),
+                                    (5000, "shared"), (700, "backward")])
+def test_getrs_interchange_lists(dev, n, kind):
+    """getrs with unit factors (L = U = I) applies exactly the interchange
+    list: rows of B swapped k <-> piv[k] in order, for lists with far,
+    near and shared targets, and (not an LU list) some piv[k] < k."""
+    print(f"getrs interchanges n={n} {kind}", flush=True)
+    r = np.random.default_rng(SEED + n)
+    if kind == "far":
+        piv = np.array([r.integers(k, n) for k in range(n)])
+    elif kind == "near":
+        piv = np.minimum(np.arange(n) + r.integers(0, 3, n), n - 1)
+    elif kind == "shared":
+        tg = r.integers(0, n, 7)
+        piv = np.maximum(np.arange(n), tg[r.integers(0, 7, n)])
+    else:
+        piv = r.integers(0, n, n)
+    B = np.arange(n, dtype=np.float64).reshape(n, 1) + 0.5
+    X = to_dev(B)
+    tmv_amd.getrs(to_dev(np.eye(n)), torch.from_numpy(piv.astype(np.int64)).cuda(), X)
+    assert np.array_equal(to_host(X), _swapped(piv, B))

@@ -703,30 +703,82 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
     return TMVX_OK;
 }
 
-// Interchange list -> composed permutation, one thread over an LDS array
-// (n <= PERM_LDS_MAX), else over global memory.
+// Interchange list -> composed permutation.  Step k swaps positions k and
+// piv[k] >= k, so position k is final after step k.  With the positions'
+// contents in LDS (n <= PERM_LDS_MAX), one wave takes the steps 64 at a
+// time: lane i holds the content of position k0 + i and that of its target
+// piv[k0 + i] when the target lies past the group (lanes sharing a target
+// share the lowest such lane's copy); the 64 swaps run on registers
+// (readlane and selects), the group's positions are then final and written
+// out, and the targets past the group go back to LDS.  Larger n, or a list
+// with some piv[k] < k (not an LU interchange list): one thread, in order.
 constexpr int PERM_LDS_MAX = 32768;
 
 __global__ void __launch_bounds__(1024)
 piv_to_perm_kernel(i64 n, const i64* __restrict__ piv, i64* __restrict__ perm)
 {
     __shared__ int sp[PERM_LDS_MAX];
+    __shared__ int s_back;               // some piv[k] < k: steps in order, one thread
     const bool lds = n <= PERM_LDS_MAX;
+    if (threadIdx.x == 0) s_back = 0;
+    __syncthreads();
     for (i64 i = threadIdx.x; i < n; i += blockDim.x) {
         if (lds) sp[i] = (int)i; else perm[i] = i;
+        const i64 p = piv[i];
+        if (p >= 0 && p < i) s_back = 1;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (i64 k = 0; k < n; ++k) {
-            const i64 p = piv[k];
-            if (p == k || p < 0 || p >= n) continue;
-            if (lds) { const int x = sp[k]; sp[k] = sp[p]; sp[p] = x; }
-            else { const i64 x = perm[k]; perm[k] = perm[p]; perm[p] = x; }
+    if (!lds || s_back) {
+        if (threadIdx.x == 0) {
+            for (i64 k = 0; k < n; ++k) {
+                const i64 p = piv[k];
+                if (p == k || p < 0 || p >= n) continue;
+                if (lds) { const int x = sp[k]; sp[k] = sp[p]; sp[p] = x; }
+                else { const i64 x = perm[k]; perm[k] = perm[p]; perm[p] = x; }
+            }
         }
+        __syncthreads();
+        if (lds)
+            for (i64 i = threadIdx.x; i < n; i += blockDim.x) perm[i] = sp[i];
+        return;
     }
-    __syncthreads();
-    if (lds)
-        for (i64 i = threadIdx.x; i < n; i += blockDim.x) perm[i] = sp[i];
+    if (threadIdx.x >= 64) return;
+    const int lane = threadIdx.x;
+    for (i64 k0 = 0; k0 < n; k0 += 64) {
+        const int cnt = (int)((n - k0) < 64 ? (n - k0) : 64);
+        const i64 k = k0 + lane;
+        const bool act = lane < cnt;
+        i64 pk = act ? piv[k] : k;
+        if (pk < k || pk >= n) pk = k;                    // out of range: no swap
+        const int p = (int)pk;
+        const bool inside = pk < k0 + 64;
+        int cv = act ? sp[k] : 0;                          // content of position k
+        int ov = (act && !inside) ? sp[p] : 0;             // content of target p
+        // owner of an outside target: the lowest lane with the same target
+        int owner = lane;
+        for (int j = 63; j >= 0; --j) {
+            const int pj = __builtin_amdgcn_readlane(p, j);
+            const int oj = __builtin_amdgcn_readlane((int)(act && !inside), j);
+            if (oj && act && !inside && pj == p) owner = j;
+        }
+        for (int i = 0; i < cnt; ++i) {
+            const int pi = __builtin_amdgcn_readlane(p, i);
+            const int a = __builtin_amdgcn_readlane(cv, i);
+            if (pi < k0 + 64) {                            // both positions in the group
+                const int bl = pi - (int)k0;
+                const int b = __builtin_amdgcn_readlane(cv, bl);
+                cv = lane == i ? b : cv;
+                cv = lane == bl ? a : cv;
+            } else {
+                const int o = __builtin_amdgcn_readlane(owner, i);
+                const int b = __builtin_amdgcn_readlane(ov, o);
+                cv = lane == i ? b : cv;
+                ov = lane == o ? a : ov;
+            }
+        }
+        if (act) perm[k] = cv;
+        if (act && !inside && owner == lane) sp[p] = ov;
+    }
 }
 
 int piv_to_perm(i64 n, const i64* piv, i64* perm, hipStream_t s)
