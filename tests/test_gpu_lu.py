@@ -77,7 +77,7 @@ def test_det_fixtures_gpu(orc, dev):
 
 
 @pytest.mark.parametrize("n,nrhs", [(1, 1), (4, 1), (33, 3), (100, 1), (257, 16), (512, 1),
-                                    (1000, 16)])
+                                    (1000, 16), (300, 2), (200, 6), (130, 7), (3000, 1)])
 @pytest.mark.parametrize("trans", [False, True])
 def test_getrs(orc, dev, n, nrhs, trans):
     print(f"getrs n={n} nrhs={nrhs} trans={trans}", flush=True)
@@ -137,6 +137,22 @@ def test_sgetrf(orc, dev, n):
     assert np.array_equal(P[:k0], Po[:k0])
     if n <= 300:
         assert lu_residual(A, LU.astype(np.float32), P) <= 8 * n * EPS32
+
+
+@pytest.mark.parametrize("nrhs", [1, 2, 3, 5])
+@pytest.mark.parametrize("trans", [False, True])
+def test_sgetrs(dev, nrhs, trans):
+    """FP32 solve (the refinement's inner solve): backward error in FP32."""
+    n = 1000
+    A = rng.uniform_matrix(SEED + 710, n, n).astype(np.float32)
+    B = rng.uniform_matrix(SEED + 711, n, nrhs).astype(np.float32)
+    Ad, LU, P, info = gpu_getrf(A, torch.float32)
+    Bd = to_dev(B, torch.float32)
+    tmv_amd.getrs(Ad, torch.from_numpy(P).cuda(), Bd, trans=trans)
+    X = to_host(Bd).astype(np.float64)
+    At = (A.T if trans else A).astype(np.float64)
+    for j in range(nrhs):
+        assert backward_error(At, X[:, j], B[:, j].astype(np.float64)) <= 8 * n * EPS32
 
 
 def test_getrf_8192_backward_error(dev):
