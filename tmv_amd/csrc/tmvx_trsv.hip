@@ -203,9 +203,11 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
     // Diagonal block: lane i keeps row i of op(A)[b, b].  Step k: x_k is
     // scaled by the reciprocal of its diagonal (a zero x_k stays as it is;
     // a zero diagonal under a non-zero x_k is singular) and broadcast, and
-    // the rows still to come subtract op(A)(i, k) x_k.  Written with selects
-    // rather than branches: the 64 steps are the dependency chain.
-    T x = acc;
+    // the rows still to come subtract op(A)(i, k) x_k.  The 64 steps are
+    // the dependency chain, so it carries no branch and no select: every
+    // lane takes the subtraction (rows already solved hold their result in
+    // xs and their x is not read again).
+    T x = acc, xs = acc;
     bool sing = false;
     auto dstep = [&](auto kc) __attribute__((always_inline)) {
         constexpr int k = decltype(kc)::value;
@@ -213,9 +215,8 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
         const T rk = readlane(rdiag, k);      // 1 for a unit diagonal
         sing |= (xr != T(0)) & (rk == T(0));
         const T xk = xr != T(0) ? xr * rk : xr;
-        const T upd = x - drow[k] * xk;
-        const bool later = lower ? lane > k : lane < k;
-        x = lane == k ? xk : (later ? upd : x);
+        x -= drow[k] * xk;
+        if (lane == k) xs = xk;
     };
     if (lower) {
         if (bs == NB)
@@ -238,6 +239,7 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
     }
     if (sing) atomicOr(wk.singular, 1);
     TRSV_TRACE(q, 3);
+    x = xs;
     if (row_ok) {
         sv_publish(xcol + (b0 + lane) * W, x, wk.epoch);
         TRSV_TRACE(q, 4);
