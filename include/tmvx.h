@@ -91,8 +91,10 @@ int tmvx_getrf_f32(int64_t n, float* A, int64_t lda, int64_t* piv,
  * B^T, src/TMV_LUDiv.cpp:369-410).  B is n x nrhs column-major, overwritten
  * with X.  A zero U diagonal met by a non-zero entry reports
  * TMVX_ERR_SINGULAR (TMV throws SingularUpperTriMatrix,
- * src/TMV_TriDiv_V.cpp:171-217); check_singular = 0 skips that check and
- * the synchronisation it needs.
+ * src/TMV_TriDiv_V.cpp:171-217), and an expired bounded device-side wait
+ * TMVX_ERR_TIMEOUT; both need a synchronisation.  check_singular = 0 skips
+ * the synchronisation and with it BOTH reports: the call is then fully
+ * asynchronous and a singular factor leaves inf/NaN in B.
  * Replaces: tmv::LU_LDivEq / LU_RDivEq (src/TMV_LUDiv.h:29-35) and the
  * dgetrs seam (src/TMV_LUDiv.cpp:58-92).
  */
@@ -227,6 +229,10 @@ int tmvx_logdet_f32(int64_t n, const float* LU, int64_t lda, const int64_t* piv,
 /*
  * B := op(A)^-1 B with A n x n triangular (upper != 0: upper), unit != 0
  * for an implicit unit diagonal, trans != 0 for A^T.  B is n x nrhs.
+ * Entries are divided by the diagonal as TMV does.  A zero diagonal met by
+ * a non-zero entry reports TMVX_ERR_SINGULAR (TMV throws
+ * SingularUpperTriMatrix / SingularLowerTriMatrix, src/TMV_TriDiv_V.cpp:
+ * 171-217).  Synchronises.
  * Replaces: tmv::TriLDivEq(Tri, MatrixView) (include/tmv/TMV_TriDiv.h:72-75,
  * src/TMV_TriDiv_M.cpp:190-266) and the dtrsm seam (src/TMV_TriDiv_M.cpp:
  * 268-310).
@@ -301,7 +307,10 @@ int tmvx_gemm_f32_lowp(int lowprec, int64_t m, int64_t n, int64_t k, float alpha
  *   gradA = -gradB X^T (+ gradA when accumulate != 0)      (n x n)
  * No counterpart in TMV, which has no derivatives; the transposed solve
  * is TMV's LU_RDivEq (x % A, src/TMV_LUDiv.cpp:369-410) and the rank-nrhs
- * update its MultMM (src/TMV_MultMM.cpp:404-477).  Asynchronous.
+ * update its MultMM (src/TMV_MultMM.cpp:404-477).  Synchronises: the
+ * transposed solve reports a singular factor (TMVX_ERR_SINGULAR) or an
+ * expired bounded wait (TMVX_ERR_TIMEOUT) as tmvx_getrs_* with
+ * check_singular != 0 does, before gradA is formed.
  */
 int tmvx_gesv_adjoint_f64(int64_t n, int64_t nrhs, const double* LU, int64_t lda,
                           const int64_t* piv, const double* X, int64_t ldx, const double* G,

@@ -94,9 +94,11 @@ int tmvx_gemm_batched_f32(int64_t count, const int32_t* m, const int32_t* n, con
 
 /*
  * Factor a batch that may hold systems above TMVX_BATCH_NMAX: n_host (host)
- * gives the orders; systems with n <= 128 run in one batched launch, larger
- * ones through the blocked LU (tmvx_getrf_*), one after another.  Pivots and
- * info as tmvx_gesv_batched_*.  Allocates and synchronises; not capturable.
+ * gives the orders; systems with n <= 128 run in one batched launch, those
+ * with 128 < n <= 1024 in one launch of a workgroup per system, larger ones
+ * through the blocked LU (tmvx_getrf_*), one after another on `stream`.
+ * Pivots and info as tmvx_gesv_batched_*.  Allocates and synchronises; not
+ * capturable.
  * Per system TMV LU_Decompose (src/TMV_LUDecompose.cpp:514-559).
  */
 int tmvx_getrf_batched_f64(int64_t count, const int32_t* n_host, double* const* A,

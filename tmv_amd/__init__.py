@@ -189,6 +189,48 @@ def _colmajor_ld(t, name: str) -> int:
     return ld
 
 
+def _on_gpu(*ts) -> None:
+    """Every tensor a device tensor, all on one device (the library takes
+    device pointers; a host pointer would be read by the GPU)."""
+    d = ts[0].device
+    for t in ts:
+        if t.device.type != "cuda":
+            raise ValueError(f"device tensors only (got a {t.device.type} tensor)")
+        if t.device != d:
+            raise ValueError(f"tensors on different devices ({d} and {t.device})")
+
+
+def _square(t, name: str) -> int:
+    if t.dim() != 2 or t.shape[0] != t.shape[1]:
+        raise ValueError(f"{name} must be a square 2-D tensor (got shape {tuple(t.shape)})")
+    return t.shape[0]
+
+
+def _check_piv(piv, n: int) -> None:
+    torch = _torch()
+    if piv.dtype != torch.int64:
+        raise TypeError(f"piv must be int64 (got {piv.dtype})")
+    if piv.dim() != 1 or piv.numel() < n or (n > 0 and piv.stride(0) != 1):
+        raise ValueError(f"piv must be a contiguous 1-D tensor of at least {n} entries")
+
+
+def _rhs(B, n: int, dtype, name: str = "B"):
+    """(B as n x nrhs, its leading dimension) for a 1-D (one column) or
+    column-major 2-D right-hand side of n rows and dtype `dtype`."""
+    if B.dim() not in (1, 2):
+        raise ValueError(f"{name} must be 1-D or 2-D")
+    if B.shape[0] != n:
+        raise ValueError(f"{name} has {B.shape[0]} rows, expected {n}")
+    if B.dtype != dtype:
+        raise TypeError(f"{name} is {B.dtype}, expected {dtype}")
+    Bm = B.view(n, 1) if B.dim() == 1 else B
+    if Bm.shape[1] > 1:
+        return Bm, _colmajor_ld(Bm, name)
+    if n > 1 and Bm.stride(0) != 1:
+        raise ValueError(f"{name} must have unit row stride")
+    return Bm, max(n, 1)
+
+
 def _suffix(dtype) -> str:
     torch = _torch()
     if dtype == torch.float64:
@@ -206,20 +248,24 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, stream=None):
     """C := alpha*A*B + beta*C (TMV MultMM<add> with beta = add).  A and B
     may be column- or row-major; C (allocated column-major when None) must
     be column-major.  Returns C."""
-    torch = _torch()
+    if A.dim() != 2 or B.dim() != 2:
+        raise ValueError("A and B must be 2-D")
     m, k = A.shape
     k2, n = B.shape
     if k != k2:
         raise ValueError("inner dimensions differ")
+    suf = _suffix(A.dtype)
+    if B.dtype != A.dtype or (C is not None and C.dtype != A.dtype):
+        raise TypeError("A, B and C must share a dtype")
+    if C is not None and tuple(C.shape) != (m, n):
+        raise ValueError(f"C has shape {tuple(C.shape)}, expected {(m, n)}")
     la, lda = _layout(A, "A")
     lb, ldb = _layout(B, "B")
+    _on_gpu(A, B, *(() if C is None else (C,)))
     if C is None:
         C = empty_colmajor(m, n, A.dtype, A.device)
         beta = 0.0
     ldc = _colmajor_ld(C, "C")
-    suf = _suffix(A.dtype)
-    if B.dtype != A.dtype or C.dtype != A.dtype:
-        raise TypeError("A, B and C must share a dtype")
     fn = getattr(lib(), "tmvx_gemm_" + suf)
     _check(fn(la, lb, m, n, k, alpha, A.data_ptr(), lda, B.data_ptr(), ldb, beta,
               C.data_ptr(), ldc, _stream(stream)), "gemm")
@@ -245,6 +291,9 @@ def gemm_complex(A, B, C=None, alpha=1.0, beta=0.0, conj_a=False, conj_b=False, 
         beta = 0.0
     if C.dtype != A.dtype:
         raise TypeError("C must have A's dtype")
+    if tuple(C.shape) != (m, n):
+        raise ValueError(f"C has shape {tuple(C.shape)}, expected {(m, n)}")
+    _on_gpu(A, B, C)
     ldc = _colmajor_ld(C, "C")
     alpha, beta = complex(alpha), complex(beta)
     fn = lib().tmvx_gemm_z if A.dtype == torch.complex128 else lib().tmvx_gemm_c
@@ -259,13 +308,13 @@ def getrf(A, stream=None):
     Returns (piv, info): piv an int64 device tensor of 0-based interchanges,
     info the 1-based index of the first zero pivot (0 if none)."""
     torch = _torch()
-    n = A.shape[0]
-    if A.shape[1] != n:
-        raise ValueError("square matrices only")
+    n = _square(A, "A")
+    suf = _suffix(A.dtype)
     lda = _colmajor_ld(A, "A")
+    _on_gpu(A)
     piv = torch.empty(n, dtype=torch.int64, device=A.device)
     info = _i64(0)
-    fn = getattr(lib(), "tmvx_getrf_" + _suffix(A.dtype))
+    fn = getattr(lib(), "tmvx_getrf_" + suf)
     _check(fn(n, A.data_ptr(), lda, piv.data_ptr(), ctypes.byref(info), _stream(stream)),
            "getrf")
     return piv, int(info.value)
@@ -275,12 +324,11 @@ def getrf_complex(A, stream=None):
     """In-place PA = LU of a square column-major complex128 / complex64
     tensor (TMV LU_Decompose for complex T).  Returns (piv, info)."""
     torch = _torch()
-    n = A.shape[0]
-    if A.shape[1] != n:
-        raise ValueError("square matrices only")
-    lda = _colmajor_ld(A, "A")
+    n = _square(A, "A")
     if A.dtype not in (torch.complex128, torch.complex64):
         raise TypeError("complex128 or complex64 only")
+    lda = _colmajor_ld(A, "A")
+    _on_gpu(A)
     piv = torch.empty(n, dtype=torch.int64, device=A.device)
     info = _i64(0)
     fn = lib().tmvx_getrf_z if A.dtype == torch.complex128 else lib().tmvx_getrf_c
@@ -292,10 +340,13 @@ def getrf_complex(A, stream=None):
 def getrs_complex(LU, piv, B, trans=False, stream=None):
     """In-place complex solve with the factors: B := A^-1 B (trans: A^-T B)."""
     torch = _torch()
-    n = LU.shape[0]
+    n = _square(LU, "LU")
+    if LU.dtype not in (torch.complex128, torch.complex64):
+        raise TypeError("complex128 or complex64 only")
     lda = _colmajor_ld(LU, "LU")
-    Bm = B.view(n, 1) if B.dim() == 1 else B
-    ldb = _colmajor_ld(Bm, "B") if Bm.shape[1] > 1 else max(n, 1)
+    _check_piv(piv, n)
+    Bm, ldb = _rhs(B, n, LU.dtype)
+    _on_gpu(LU, piv, B)
     fn = lib().tmvx_getrs_z if LU.dtype == torch.complex128 else lib().tmvx_getrs_c
     _check(fn(int(bool(trans)), n, Bm.shape[1], LU.data_ptr(), lda, piv.data_ptr(),
               Bm.data_ptr(), ldb, _stream(stream)), "getrs_complex")
@@ -318,10 +369,13 @@ def geqrs(QR, beta, B, stream=None):
     """QR division: B (m x nrhs) := Q^T B, then its first n rows := R^-1 of
     themselves (least squares for m > n).  Returns B."""
     m, n = QR.shape
+    suf = _suffix(QR.dtype)
     lda = _colmajor_ld(QR, "QR") if n > 1 else max(m, 1)
-    Bm = B.view(m, 1) if B.dim() == 1 else B
-    ldb = _colmajor_ld(Bm, "B") if Bm.shape[1] > 1 else max(m, 1)
-    fn = getattr(lib(), "tmvx_geqrs_" + _suffix(QR.dtype))
+    if beta.dtype != QR.dtype or beta.dim() != 1 or beta.numel() < n:
+        raise ValueError(f"beta must be a 1-D {QR.dtype} tensor of at least {n} entries")
+    Bm, ldb = _rhs(B, m, QR.dtype)
+    _on_gpu(QR, beta, B)
+    fn = getattr(lib(), "tmvx_geqrs_" + suf)
     _check(fn(m, n, Bm.shape[1], QR.data_ptr(), lda, beta.data_ptr(), Bm.data_ptr(), ldb,
               _stream(stream)), "geqrs")
     return B
@@ -330,8 +384,12 @@ def geqrs(QR, beta, B, stream=None):
 def logdet_complex(LU, piv, stream=None):
     """(log|det A|, complex sign) from complex128 factors (TMV LUDiv::logDet
     for complex T); sign 0 and logdet -inf for a singular matrix."""
-    n = LU.shape[0]
+    n = _square(LU, "LU")
+    if LU.dtype != _torch().complex128:
+        raise TypeError("complex128 only")
     lda = _colmajor_ld(LU, "LU")
+    _check_piv(piv, n)
+    _on_gpu(LU, piv)
     ld, sr, si = _d(0.0), _d(0.0), _d(0.0)
     _check(lib().tmvx_logdet_z(n, LU.data_ptr(), lda, piv.data_ptr(), ctypes.byref(ld),
                                ctypes.byref(sr), ctypes.byref(si), _stream(stream)),
@@ -342,8 +400,12 @@ def logdet_complex(LU, piv, stream=None):
 def getri_complex(LU, piv, stream=None):
     """A^-1 from complex128 factors; returns a new column-major tensor."""
     torch = _torch()
-    n = LU.shape[0]
+    n = _square(LU, "LU")
+    if LU.dtype != torch.complex128:
+        raise TypeError("complex128 only")
     lda = _colmajor_ld(LU, "LU")
+    _check_piv(piv, n)
+    _on_gpu(LU, piv)
     X = colmajor(torch.empty(n, n, dtype=LU.dtype, device=LU.device))
     _check(lib().tmvx_getri_z(n, LU.data_ptr(), lda, piv.data_ptr(), X.data_ptr(), max(n, 1),
                               _stream(stream)), "getri_complex")
@@ -354,10 +416,11 @@ def getrf_tournament(A, stream=None):
     """FP32 PA = LU with tournament pivoting (the mixed solver's panel
     factorisation).  Returns (piv, info) like getrf."""
     torch = _torch()
-    n = A.shape[0]
-    lda = _colmajor_ld(A, "A")
+    n = _square(A, "A")
     if A.dtype != torch.float32:
         raise TypeError("float32 only")
+    lda = _colmajor_ld(A, "A")
+    _on_gpu(A)
     piv = torch.empty(n, dtype=torch.int64, device=A.device)
     info = _i64(0)
     _check(lib().tmvx_getrf_f32_tournament(n, A.data_ptr(), lda, piv.data_ptr(),
@@ -368,11 +431,13 @@ def getrf_tournament(A, stream=None):
 def getrs(LU, piv, B, trans=False, check_singular=True, stream=None):
     """In-place solve with the factors: B := A^-1 B (trans: A^-T B).
     B is column-major n x nrhs (a 1-D tensor is treated as n x 1)."""
-    n = LU.shape[0]
+    n = _square(LU, "LU")
+    suf = _suffix(LU.dtype)
     lda = _colmajor_ld(LU, "LU")
-    Bm = B.view(n, 1) if B.dim() == 1 else B
-    ldb = _colmajor_ld(Bm, "B") if Bm.shape[1] > 1 else max(n, 1)
-    fn = getattr(lib(), "tmvx_getrs_" + _suffix(LU.dtype))
+    _check_piv(piv, n)
+    Bm, ldb = _rhs(B, n, LU.dtype)
+    _on_gpu(LU, piv, B)
+    fn = getattr(lib(), "tmvx_getrs_" + suf)
     _check(fn(int(bool(trans)), n, Bm.shape[1], LU.data_ptr(), lda, piv.data_ptr(),
               Bm.data_ptr(), ldb, int(bool(check_singular)), _stream(stream)), "getrs")
     return B
@@ -382,10 +447,13 @@ def getri(LU, piv, stream=None):
     """A^-1 from the factors (TMV LU_Inverse / m.inverse() with LU
     division); returns a new column-major n x n tensor."""
     torch = _torch()
-    n = LU.shape[0]
+    n = _square(LU, "LU")
+    suf = _suffix(LU.dtype)
     lda = _colmajor_ld(LU, "LU")
+    _check_piv(piv, n)
+    _on_gpu(LU, piv)
     X = colmajor(torch.empty(n, n, dtype=LU.dtype, device=LU.device))
-    fn = getattr(lib(), "tmvx_getri_" + _suffix(LU.dtype))
+    fn = getattr(lib(), "tmvx_getri_" + suf)
     _check(fn(n, LU.data_ptr(), lda, piv.data_ptr(), X.data_ptr(), max(n, 1), _stream(stream)),
            "getri")
     return X
@@ -394,10 +462,13 @@ def getri(LU, piv, stream=None):
 def logdet(LU, piv, stream=None):
     """(log|det A|, sign) from the factors (TMV LUDiv::logDet); sign is
     +1, -1, or 0 for a singular matrix (logdet -inf)."""
-    n = LU.shape[0]
+    n = _square(LU, "LU")
+    suf = _suffix(LU.dtype)
     lda = _colmajor_ld(LU, "LU")
+    _check_piv(piv, n)
+    _on_gpu(LU, piv)
     ld, sg = _d(0.0), _d(0.0)
-    fn = getattr(lib(), "tmvx_logdet_" + _suffix(LU.dtype))
+    fn = getattr(lib(), "tmvx_logdet_" + suf)
     _check(fn(n, LU.data_ptr(), lda, piv.data_ptr(), ctypes.byref(ld), ctypes.byref(sg),
               _stream(stream)), "logdet")
     return ld.value, sg.value
@@ -411,21 +482,37 @@ def det(LU, piv, stream=None):
 
 
 def trsm(A, B, upper, trans=False, unit=False, stream=None):
-    n = A.shape[0]
+    """B := op(A)^-1 B for triangular A (TMV TriLDivEq); raises
+    TmvxError(TMVX_ERR_SINGULAR) on a zero diagonal met by a non-zero
+    entry."""
+    n = _square(A, "A")
+    suf = _suffix(A.dtype)
     lda = _colmajor_ld(A, "A")
-    Bm = B.view(n, 1) if B.dim() == 1 else B
-    ldb = _colmajor_ld(Bm, "B") if Bm.shape[1] > 1 else max(n, 1)
-    fn = getattr(lib(), "tmvx_trsm_" + _suffix(A.dtype))
+    Bm, ldb = _rhs(B, n, A.dtype)
+    _on_gpu(A, B)
+    fn = getattr(lib(), "tmvx_trsm_" + suf)
     _check(fn(int(bool(upper)), int(bool(trans)), int(bool(unit)), n, Bm.shape[1],
               A.data_ptr(), lda, Bm.data_ptr(), ldb, _stream(stream)), "trsm")
     return B
 
 
 def laswp(A, piv, k1=0, k2=None, forward=True, stream=None):
+    """Interchanges piv[k1:k2] applied to the rows of A in order (TMV
+    permuteRows) or in reverse order (reversePermuteRows)."""
     m, ncols = A.shape if A.dim() == 2 else (A.shape[0], 1)
     k2 = piv.numel() if k2 is None else k2
-    lda = _colmajor_ld(A, "A") if A.dim() == 2 and ncols > 1 else max(m, 1)
-    fn = getattr(lib(), "tmvx_laswp_" + _suffix(A.dtype))
+    suf = _suffix(A.dtype)
+    if not 0 <= k1 <= k2 <= m:
+        raise ValueError(f"need 0 <= k1 <= k2 <= {m} (got k1={k1}, k2={k2})")
+    _check_piv(piv, k2)
+    if A.dim() == 2 and ncols > 1:
+        lda = _colmajor_ld(A, "A")
+    else:
+        if m > 1 and A.stride(0) != 1:
+            raise ValueError("A must have unit row stride")
+        lda = max(m, 1)
+    _on_gpu(A, piv)
+    fn = getattr(lib(), "tmvx_laswp_" + suf)
     _check(fn(int(bool(forward)), ncols, A.data_ptr(), lda, k1, k2, piv.data_ptr(),
               _stream(stream)), "laswp")
     return A
@@ -453,15 +540,20 @@ def gesv_mixed(A, B, X=None, tol=1e-12, max_iter=30, lowprec=LOWPREC_F32, check=
     not modified) factorised in FP32 on MFMA, residual and update in FP64.
     Returns (X, info dict).  Raises on failure when check is set."""
     torch = _torch()
-    n = A.shape[0]
+    n = _square(A, "A")
+    if A.dtype != torch.float64:
+        raise TypeError("A must be float64")
     lda = _colmajor_ld(A, "A")
-    Bm = B.view(n, 1) if B.dim() == 1 else B
-    ldb = _colmajor_ld(Bm, "B") if Bm.shape[1] > 1 else max(n, 1)
+    Bm, ldb = _rhs(B, n, torch.float64)
     nrhs = Bm.shape[1]
+    if X is not None:
+        Xm, ldx = _rhs(X, n, torch.float64, "X")
+        if Xm.shape[1] != nrhs:
+            raise ValueError(f"X has {Xm.shape[1]} columns, expected {nrhs}")
+    _on_gpu(A, B, *(() if X is None else (X,)))
     if X is None:
         X = empty_colmajor(n, nrhs, torch.float64, A.device)
-    Xm = X.view(n, 1) if X.dim() == 1 else X
-    ldx = _colmajor_ld(Xm, "X") if Xm.shape[1] > 1 else max(n, 1)
+        Xm, ldx = _rhs(X, n, torch.float64, "X")
     info = IrInfo()
     rc = lib().tmvx_gesv_mixed_f64(lowprec, n, nrhs, A.data_ptr(), lda, Bm.data_ptr(), ldb,
                                    Xm.data_ptr(), ldx, tol, max_iter, ctypes.byref(info),
@@ -473,15 +565,23 @@ def gesv_mixed(A, B, X=None, tol=1e-12, max_iter=30, lowprec=LOWPREC_F32, check=
     return (X if B.dim() > 1 else Xm.view(n)), d
 
 
+def _fill_target(out) -> None:
+    if out.dtype != _torch().float64 or not out.is_contiguous():
+        raise TypeError("a contiguous float64 tensor is filled")
+    _on_gpu(out)
+
+
 def fill_uniform(out, seed, lo=-1.0, hi=1.0, offset=0, stream=None):
     """Fill a contiguous float64 device buffer (flat counter order) with the
     documented SplitMix64 uniform stream (see tmv_amd.rng)."""
+    _fill_target(out)
     _check(lib().tmvx_fill_uniform_f64(seed, offset, out.numel(), lo, hi, out.data_ptr(),
                                        _stream(stream)), "fill_uniform")
     return out
 
 
 def fill_normal(out, seed, offset=0, stream=None):
+    _fill_target(out)
     _check(lib().tmvx_fill_normal_f64(seed, offset, out.numel(), out.data_ptr(),
                                       _stream(stream)), "fill_normal")
     return out
@@ -634,20 +734,25 @@ def gesv_adjoint(LU, piv, X, G, gradA=None, accumulate=False, stream=None):
     """gradB = A^-T G, gradA (+)= -gradB X^T for one system with the factors
     of getrf.  X, G n x nrhs column-major (1-D = one column).  Returns
     (gradB, gradA)."""
-    torch = _torch()
-    n = LU.shape[0]
+    n = _square(LU, "LU")
+    suf = _suffix(LU.dtype)
     lda = _colmajor_ld(LU, "LU")
-    Xm = X.view(n, 1) if X.dim() == 1 else X
-    Gm = G.view(n, 1) if G.dim() == 1 else G
+    _check_piv(piv, n)
+    Xm, ldx = _rhs(X, n, LU.dtype, "X")
+    Gm, ldg = _rhs(G, n, LU.dtype, "G")
     nrhs = Xm.shape[1]
-    ldx = _colmajor_ld(Xm, "X") if nrhs > 1 else max(n, 1)
-    ldg = _colmajor_ld(Gm, "G") if nrhs > 1 else max(n, 1)
+    if Gm.shape[1] != nrhs:
+        raise ValueError(f"G has {Gm.shape[1]} columns, X has {nrhs}")
+    if gradA is not None:
+        if gradA.dtype != LU.dtype or tuple(gradA.shape) != (n, n):
+            raise ValueError(f"gradA must be a {n} x {n} {LU.dtype} tensor")
+    _on_gpu(LU, piv, X, G, *(() if gradA is None else (gradA,)))
     gB = empty_colmajor(n, nrhs, LU.dtype, LU.device)
     if gradA is None:
         gradA = empty_colmajor(n, n, LU.dtype, LU.device)
         accumulate = False
     ldga = _colmajor_ld(gradA, "gradA")
-    fn = getattr(lib(), "tmvx_gesv_adjoint_" + _suffix(LU.dtype))
+    fn = getattr(lib(), "tmvx_gesv_adjoint_" + suf)
     _check(fn(n, nrhs, LU.data_ptr(), lda, piv.data_ptr(), Xm.data_ptr(), ldx, Gm.data_ptr(),
               ldg, gB.data_ptr(), max(n, 1), gradA.data_ptr(), ldga, int(bool(accumulate)),
               _stream(stream)), "gesv_adjoint")
@@ -682,9 +787,17 @@ def gemm_lowp(A, B, C=None, alpha=1.0, beta=0.0, lowprec=LOWPREC_F16, scale_b=Tr
     MFMA with FP32 accumulation (tmvx_gemm_f32_lowp)."""
     torch = _torch()
     m, k = A.shape
-    n = B.shape[1]
+    k2, n = B.shape
+    if k != k2:
+        raise ValueError("inner dimensions differ")
+    if A.dtype != torch.float32 or B.dtype != torch.float32 or (
+            C is not None and C.dtype != torch.float32):
+        raise TypeError("float32 operands only")
+    if C is not None and tuple(C.shape) != (m, n):
+        raise ValueError(f"C has shape {tuple(C.shape)}, expected {(m, n)}")
     lda = _colmajor_ld(A, "A")
     ldb = _colmajor_ld(B, "B")
+    _on_gpu(A, B, *(() if C is None else (C,)))
     if C is None:
         C = empty_colmajor(m, n, torch.float32, A.device)
         beta = 0.0

@@ -23,7 +23,6 @@
 
 #include <algorithm>
 #include <climits>
-#include <thread>
 #include <vector>
 
 namespace tmvx {
@@ -286,36 +285,20 @@ static int getrf_batched_impl(int64_t count, const int32_t* n_host, T* const* A,
         hipFree(dptr);
         hipFree(dinfo);
     }
-    // Systems above WG_NMAX: the blocked LU, one system per call, from up to
-    // 16 host threads with a stream each (every call synchronises its own
-    // stream, so the calls overlap on the GPU).
+    // Systems above WG_NMAX: the blocked LU, one system after another on the
+    // caller's stream (each call synchronises to read its info).  An
+    // earlier version overlapped them from a pool of host threads, each
+    // creating streams, events and stream-ordered allocations; that pool
+    // crashed the process once under a profiler and was removed: the
+    // blocked LU of one such system already spreads over the whole GPU.
     if (rc == TMVX_OK && !big.empty()) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess) return TMVX_ERR_HIP;
-        const int nth = (int)std::min<size_t>(big.size(), 16);
-        std::vector<int> trc(nth, TMVX_OK);
         std::vector<int32_t> hinfo(big.size(), 0);
-        auto work = [&](int tid) {
-            hipStream_t st = nullptr;
-            if (hipSetDevice(dev) != hipSuccess ||
-                hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
-                trc[tid] = TMVX_ERR_HIP;
-                return;
-            }
-            for (size_t k = tid; k < big.size() && trc[tid] == TMVX_OK; k += nth) {
-                const int i = big[k];
-                i64 inf = 0;
-                trc[tid] = getrf_impl<T>(n_host[i], hA[i], n_host[i], (i64*)hP[i], nullptr, &inf, st);
-                hinfo[k] = (int32_t)inf;
-            }
-            hipStreamDestroy(st);
-        };
-        std::vector<std::thread> pool;
-        for (int tid = 1; tid < nth; ++tid) pool.emplace_back(work, tid);
-        work(0);
-        for (auto& th : pool) th.join();
-        for (int tid = 0; tid < nth; ++tid)
-            if (trc[tid] != TMVX_OK) rc = trc[tid];
+        for (size_t k = 0; k < big.size() && rc == TMVX_OK; ++k) {
+            const int i = big[k];
+            i64 inf = 0;
+            rc = getrf_impl<T>(n_host[i], hA[i], n_host[i], (i64*)hP[i], nullptr, &inf, s);
+            hinfo[k] = (int32_t)inf;
+        }
         for (size_t k = 0; k < big.size() && rc == TMVX_OK; ++k)
             if (hipMemcpyAsync(info + big[k], &hinfo[k], sizeof(int32_t), hipMemcpyHostToDevice,
                                s) != hipSuccess)
@@ -393,11 +376,13 @@ static int adjoint_single(int64_t n, int64_t nrhs, const T* LU, int64_t lda, con
     if (G != gB)
         TMVX_HIP(hipMemcpy2DAsync(gB, sizeof(T) * ldgb, G, sizeof(T) * ldg, sizeof(T) * n, nrhs,
                                   hipMemcpyDeviceToDevice, s));
+    // check = 1: a singular factor or an expired bounded wait in the
+    // transposed solve is reported, not turned into a silent gradient.
     int rc;
     if constexpr (sizeof(T) == 8)
-        rc = tmvx_getrs_f64(1, n, nrhs, LU, lda, piv, gB, ldgb, 0, s);
+        rc = tmvx_getrs_f64(1, n, nrhs, LU, lda, piv, gB, ldgb, 1, s);
     else
-        rc = tmvx_getrs_f32(1, n, nrhs, LU, lda, piv, gB, ldgb, 0, s);
+        rc = tmvx_getrs_f32(1, n, nrhs, LU, lda, piv, gB, ldgb, 1, s);
     if (rc != TMVX_OK) return rc;
     // gradA (+)= -gB X^T: X (n x nrhs, column-major) read as the row-major
     // nrhs x n operand X^T

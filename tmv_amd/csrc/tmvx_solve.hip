@@ -244,16 +244,29 @@ int laswp_impl(int forward, i64 ncols, T* A, i64 lda, i64 k1, i64 k2, const i64*
     return hip_status(hipGetLastError());
 }
 
+// The public triangular solve: synchronises to report a zero diagonal met
+// by a non-zero entry as TMVX_ERR_SINGULAR (TMV throws SingularUpperTriMatrix
+// / SingularLowerTriMatrix, src/TMV_TriDiv_V.cpp:171-217).
 template <typename T>
 int trsm_public(int upper, int trans, int unit, i64 n, i64 nrhs, const T* A, i64 lda, T* B,
                 i64 ldb, hipStream_t s)
 {
+    if (n < 0 || nrhs < 0 || lda < (n > 0 ? n : 1) || ldb < (n > 0 ? n : 1))
+        return TMVX_ERR_ARG;
+    if (n == 0 || nrhs == 0) return TMVX_OK;
     int* d = nullptr;
     if (hipMallocAsync((void**)&d, sizeof(int), s) != hipSuccess) return TMVX_ERR_ALLOC;
     int rc = zero_bytes(d, sizeof(int), s);
     if (rc == TMVX_OK)
         rc = trsm_impl<T>(upper, trans, unit, n, nrhs, A, lda, B, ldb, nullptr, nullptr, d, s);
+    int hs = 0;
+    if (rc == TMVX_OK &&
+        hipMemcpyAsync(&hs, d, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = TMVX_ERR_HIP;
     hipFreeAsync(d, s);
+    if (hipStreamSynchronize(s) != hipSuccess) return TMVX_ERR_HIP;
+    if (rc == TMVX_OK && (hs & 2)) return TMVX_ERR_TIMEOUT;
+    if (rc == TMVX_OK && hs) return TMVX_ERR_SINGULAR;
     return rc;
 }
 
