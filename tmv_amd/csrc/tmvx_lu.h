@@ -58,6 +58,17 @@ template <typename T>
 int getrs_perm(int trans, i64 n, i64 nrhs, const T* LU, i64 lda, const i64* perm, T* B,
                i64 ldb, int* d_singular, hipStream_t s);
 
+// A side stream with two events for the look-ahead factorisation, taken
+// from a per-device pool (tmvx_util.hip) and returned after use; idle sets
+// have no pending work.
+struct SideSet {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_main = nullptr, ev_side = nullptr;
+    int device = 0;
+};
+int side_acquire(SideSet** out);
+void side_release(SideSet* ss);
+
 // Fill a device buffer with zeros (a kernel, so it may be graph-captured).
 int zero_bytes(void* p, size_t bytes, hipStream_t s);
 

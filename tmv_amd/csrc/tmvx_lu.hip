@@ -39,9 +39,17 @@
 namespace tmvx {
 
 constexpr int PANEL_NB = 64;        // panel width (columns)
-constexpr int PANEL_THREADS = 512;  // rows per workgroup (one per thread)
-constexpr int PANEL_GMAX = 64;      // workgroups per panel (n <= 32768)
-constexpr unsigned SPIN_LIMIT = 1u << 22;
+constexpr int PANEL_BB = 8;         // columns per delayed-update block
+// Rows per workgroup (one per thread); a build option for timing the
+// choice: -DTMVX_PANEL_THREADS=256.
+#ifndef TMVX_PANEL_THREADS
+#define TMVX_PANEL_THREADS 512
+#endif
+constexpr int PANEL_THREADS = TMVX_PANEL_THREADS;
+constexpr int PANEL_GMAX = 512;     // workgroups per panel (n <= 262144)
+// Bounded waits: a wait that sees no progress for this long (wall clock,
+// hipDeviceAttributeWallClockRate = 100 MHz on MI355X) raises the error flag.
+constexpr long long WAIT_TICKS = 200000000ll;   // 2 s
 
 // Device-coherent (agent scope) relaxed loads/stores for the exchange slots:
 // they are performed at the device coherence point, so a workgroup on any
@@ -54,23 +62,23 @@ __device__ __forceinline__ T cload(const T* p)
 { return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 struct PanelWork {
-    unsigned long long* slots;   // [2][G][SLOT] tagged words (see SlotLayout)
+    unsigned long long* cslots;  // [2][ColSlot::SIZE][G] per-column candidates
+    unsigned long long* bslots;  // [2][BB][NB * WPV] per-block pivot rows
     int* err;                    // timeout flag
     i64* first_zero;             // 1-based first zero pivot (min), n+1 if none
 };
 
-// Exchange slot of one workgroup for one column parity, in tagged 64-bit
-// words: (column tag << 32) | 32 payload bits.  A double takes two words
-// (low, high half), a float one.  Each word is written and read as one
-// 64-bit atomic, so a reader that sees the current tag in a word also sees
-// that word's payload: no separate flag and no ordering fence is needed.
-template <typename T, int NB> struct SlotLayout {
-    static constexpr int WPV = sizeof(T) / 4;             // words per value
-    static constexpr int ROW = 0;                          // candidate row
-    static constexpr int DIAG = NB * WPV;                  // row k (slab 0 only)
-    static constexpr int VAL = 2 * NB * WPV;               // candidate |value|
-    static constexpr int IDX = VAL + WPV;                  // candidate row index
-    static constexpr int SIZE = IDX + 1;
+// Per-column exchange slot of one workgroup, in tagged 64-bit words:
+// (column tag << 32) | 32 payload bits.  A double takes two words (low,
+// high half), a float one.  Each word is written and read as one 64-bit
+// atomic, so a reader that sees the current tag in a word also sees that
+// word's payload: no separate flag and no ordering fence is needed.
+template <typename T, int BB> struct ColSlot {
+    static constexpr int WPV = sizeof(T) / 4;   // words per value
+    static constexpr int VAL = 0;               // candidate |value|
+    static constexpr int POS = WPV;             // candidate row position
+    static constexpr int ROW = WPV + 1;         // candidate's BB block values
+    static constexpr int SIZE = ROW + BB * WPV;
 };
 
 template <typename T>
@@ -93,261 +101,433 @@ __device__ __forceinline__ T from_halves(const unsigned* h)
         return __uint_as_float(h[0]);
 }
 
-// Phase timestamps of the panel loop (shader clock), only in a build with
-// -DTMVX_PANEL_TRACE: workgroup 0, thread 0, first traced panel.
+// Timestamps of the panel loop, only in a build with -DTMVX_PANEL_TRACE,
+// for the panel starting at column g_panel_trace_j0: per column, thread 0
+// of every workgroup records the wall clock (100 MHz, common to all CUs)
+// after the barrier, at publication, when every slot has arrived and after
+// the update: g_panel_trace[(k * G + w) * 4 + phase].
 #ifdef TMVX_PANEL_TRACE
 __device__ unsigned long long* g_panel_trace;
+__device__ long long g_panel_trace_j0;
+#define PANEL_TRACE_INIT()                                                     \
+    unsigned long long* const trace_buf =                                      \
+        (t == 0 && j0 == g_panel_trace_j0) ? g_panel_trace : nullptr
 #define PANEL_TRACE(k, ph)                                                     \
     do {                                                                       \
-        if (g_panel_trace && w == 0 && t == 0 && j0 == 0)                      \
-            g_panel_trace[(k) * 8 + (ph)] = clock64();                         \
+        if (trace_buf) trace_buf[((k) * G + w) * 4 + (ph)] = wall_clock64();   \
     } while (0)
 #else
+#define PANEL_TRACE_INIT() do {} while (0)
 #define PANEL_TRACE(k, ph) do {} while (0)
 #endif
 
-template <typename T, int NB, int NTH>
+// One panel [j0, j0+jb) x rows [j0, n): GEPP with TMV's pivot rules.
+//
+// Rows never move while the panel is factored: each thread keeps its row
+// in registers together with the row's current position in the swapped
+// order (pos) and whether it has become a pivot row (used).  At step k the
+// candidates are the unused rows (exactly those at positions >= k); the
+// winner p (largest |a|, then lowest position: TMV's strict '>') takes
+// position k and the row that was at k takes p's old position, which is
+// all a row interchange changes.  At the end every row is written to its
+// final position, so the interchanges on the panel's own columns cost
+// nothing extra.
+//
+// Updates are delayed by blocks of BB columns: a step updates only the
+// current block's columns of the unused rows; at the end of a block every
+// workgroup receives the block's BB pivot rows, forms U12 = L11^-1 A12 for
+// the columns to the right, and updates its rows with one rank-BB product.
+// The per-column exchange therefore carries only (|value|, position, BB
+// block values) per workgroup.  Operation order per element is that of the
+// unblocked right-looking elimination, so the factors do not depend on BB.
+//
+// Registers hold the row rotated: by whole blocks at block ends (before
+// block kb, row[c] is panel column (kb*BB + c) mod NB for c >= BB), and
+// within the block by one column per step (the current column is always
+// row[0]), so the column loop is one copy of code with static register
+// indices.
+//
+// Per column: each wave's argmax goes to LDS (double-buffered by column
+// parity), one barrier, wave 0 reduces them, publishes the workgroup's
+// candidate, receives every workgroup's slot (lane q holds slot q), picks
+// the winner and puts it in LDS; a second barrier hands it to every wave.
+template <typename T, int NB, int BB, int NTH>
 __global__ void __launch_bounds__(NTH)
-panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb,
-             i64* __restrict__ piv, PanelWork wk)
+panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb, i64* __restrict__ piv,
+             PanelWork wk)
 {
-    typedef SlotLayout<T, NB> SL;
-    constexpr int WPV = SL::WPV;
+    typedef ColSlot<T, BB> CS;
+    constexpr int WPV = CS::WPV;
+    constexpr int NW = NTH / 64;
+    constexpr int NBLK = NB / BB;
+    constexpr int RW = NB * WPV;                 // words of one staged row
+    constexpr int BW = BB * WPV;                 // words of a candidate's block values
+    static_assert(NB % BB == 0 && BB % 4 == 0 && NW <= 8 && CS::SIZE <= 64, "panel shape");
     const int G = gridDim.x, w = blockIdx.x, t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    constexpr int NW = NTH / 64;
-    const i64 mp = n - j0;
-    const int r = w * NTH + t;                 // panel-relative row
-    const bool own = r < mp;
+    const i64 m = n - j0;
+    const int r = w * NTH + t;                   // panel-relative row
+    const bool own = r < m;
 
-    // Received payload halves: candidate rows of every slab, values,
-    // indices, and row k.
-    __shared__ unsigned s_cand[PANEL_GMAX][NB * WPV];
-    __shared__ unsigned s_diag[NB * WPV];
-    __shared__ unsigned s_cval[PANEL_GMAX][WPV];
-    __shared__ int s_cidx[PANEL_GMAX];
-    __shared__ __attribute__((aligned(16))) T s_out[2][NB];   // rows to publish (cand, diag)
-    __shared__ __attribute__((aligned(16))) T s_prow[NB];   // winning row
-    __shared__ __attribute__((aligned(16))) T s_orow[NB];   // row k before the swap
-    __shared__ T s_wv[NW];
-    __shared__ int s_wi[NW];
-    __shared__ T s_bv;
-    __shared__ int s_p, s_win, s_zero, s_fail;
+    __shared__ T s_wv[2][NW];                    // per-wave candidates (column parity)
+    __shared__ int s_wi[2][NW];
+    __shared__ __attribute__((aligned(16))) T s_wrow[2][NW][BB];   // their block values
+    __shared__ __attribute__((aligned(16))) T s_pv[BB];            // the pivot row's block values
+    __shared__ int s_p, s_zero, s_fail;
+    __shared__ int s_own[BB];                    // block pivot i is held here
+    // block pivot rows (block order), as 32-bit words: staged by their
+    // owners or received; U12 is solved in place in columns >= BB
+    __shared__ __attribute__((aligned(16))) unsigned s_rw[BB * RW];
+    T (*s_rows)[NB] = reinterpret_cast<T (*)[NB]>(s_rw);
+
+    // Panel waves go first on SIMDs they share with other kernels' waves.
+    __builtin_amdgcn_s_setprio(3);
+    PANEL_TRACE_INIT();
 
     T row[NB];
-    T* gp = A + (j0 + r) + j0 * lda;
+    {
+        const T* gp = A + (j0 + r) + j0 * lda;
 #pragma unroll
-    for (int c = 0; c < NB; ++c) row[c] = (own && c < jb) ? gp[(i64)c * lda] : T(0);
+        for (int c = 0; c < NB; ++c) row[c] = (own && c < jb) ? gp[(i64)c * lda] : T(0);
+    }
+    int pos = r;                  // current position (panel-relative)
+    bool used = !own;             // a pivot row (or no row at all)
+    int mypiv = -1;               // index of this row among the current block's pivots
+    bool dfr = false;             // takes the previous block's deferred update chunks
+    int nchunk = 0;               // deferred chunks pending (uniform)
+    if (t == 0) s_fail = 0;
+    if (t < BB) s_own[t] = 0;
+    bool alive = __hip_atomic_load(wk.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    int nrot = 0;                 // whole-block rotations so far
 
-    // Rows are kept rotated: before column k is processed, row[c] holds
-    // panel column (k + c) mod NB, so the current column is always row[0]
-    // and every register index is static.  Each step rotates by one; the
-    // published candidate and diagonal rows share the rotation.
-    auto rotate = [&]() {
+    auto rotate_blocks = [&]() __attribute__((always_inline)) {
+        T first[BB];
+#pragma unroll
+        for (int q = 0; q < BB; ++q) first[q] = row[q];
+#pragma unroll
+        for (int c = 0; c < NB - BB; ++c) row[c] = row[c + BB];
+#pragma unroll
+        for (int q = 0; q < BB; ++q) row[NB - BB + q] = first[q];
+        ++nrot;
+    };
+    auto rotate_in_block = [&]() __attribute__((always_inline)) {
         const T first = row[0];
 #pragma unroll
-        for (int c = 0; c < NB - 1; ++c) row[c] = row[c + 1];
-        row[NB - 1] = first;
+        for (int q = 0; q < BB - 1; ++q) row[q] = row[q + 1];
+        row[BB - 1] = first;
     };
-
-    if (t == 0) s_fail = 0;
-    T cv = own ? fabs(row[0]) : T(-1);    // candidate for column 0
-    bool alive = true;
-    int nrot = 0;
-
-    for (int k = 0; k < jb && alive; ++k) {
-        const int par = k & 1;
-        const unsigned long long tag = (unsigned long long)(j0 + k + 1) << 32;
-        unsigned long long* myslot = wk.slots + ((i64)par * G + w) * SL::SIZE;
-        PANEL_TRACE(k, 0);
-        // -- local argmax over rows >= k --------------------------------
-        T v = (own && r >= k) ? cv : T(-1);
-        int vi = r;
-        wave_argmax(v, vi);
-        if (lane == 0) { s_wv[wave] = v; s_wi[wave] = vi; }
-        __syncthreads();
-        if (wave == 0) {                 // across waves: one more DPP argmax
-            T bv = lane < NW ? s_wv[lane] : T(-1);
-            int bi = lane < NW ? s_wi[lane] : INT_MAX;
-            wave_argmax(bv, bi);
-            if (lane == 0) { s_bv = bv; s_p = bi; }
-        }
-        __syncthreads();
-        PANEL_TRACE(k, 1);
-        // -- publish the candidate row and (slab 0) row k as tagged words --
+    // Deferred share of the previous block's rank-BB update: chunk j (the
+    // BB columns now at row[j*BB ..]) with the block's multipliers (now at
+    // row[NB-BB ..]) and its U12 rows (still in s_rows, columns (j+1)*BB ..).
+    // Applied one chunk per column step while the workgroup waits for the
+    // exchange; all are done before the block's own end.
+    auto deferred_chunk = [&](int kk) __attribute__((always_inline)) {
+        if (kk >= nchunk || !dfr) return;
         typedef typename Vec16<T>::type VT;
         constexpr int VN = Vec16<T>::n;
-        if (own && r == s_p) {          // 16-byte LDS stores
-            static_for<0, NB / VN>([&](auto cc) {
-                constexpr int c = decltype(cc)::value * VN;
-                VT x;
-                static_for<0, VN>([&](auto uc) { x[decltype(uc)::value] = row[c + decltype(uc)::value]; });
-                *reinterpret_cast<VT*>(&s_out[0][c]) = x;
-            });
-        }
-        if (own && r == k) {
-            static_for<0, NB / VN>([&](auto cc) {
-                constexpr int c = decltype(cc)::value * VN;
-                VT x;
-                static_for<0, VN>([&](auto uc) { x[decltype(uc)::value] = row[c + decltype(uc)::value]; });
-                *reinterpret_cast<VT*>(&s_out[1][c]) = x;
-            });
-        }
-        __syncthreads();
-        for (int e = t; e < SL::SIZE; e += NTH) {
-            unsigned pay;
-            if (e < SL::DIAG)      pay = half_bits(s_out[0][e / WPV], e % WPV);
-            else if (e < SL::VAL) {
-                if (w != 0) continue;                       // row k: slab 0 only
-                pay = half_bits(s_out[1][(e - SL::DIAG) / WPV], (e - SL::DIAG) % WPV);
-            }
-            else if (e < SL::IDX)  pay = half_bits(s_bv, e - SL::VAL);
-            else                   pay = (unsigned)s_p;
-            cstore(myslot + e, tag | pay);
-        }
-        PANEL_TRACE(k, 2);
-        // -- receive every slab's words once they carry this column's tag --
-        //    Each thread issues all of its loads, then re-polls only the
-        //    words that are still stale: one round trip per poll, not one
-        //    per word.
-        {
-            const int nrow = G * SL::DIAG;               // candidate-row words
-            const int total = nrow + SL::DIAG + G * (WPV + 1);
-            constexpr int CW = sizeof(T) == 8 ? 4 : 8;   // words per thread per poll
-            auto src_of = [&](int e) -> const unsigned long long* {
-                if (e < nrow)
-                    return wk.slots + ((i64)par * G + e / SL::DIAG) * SL::SIZE + SL::ROW +
-                           e % SL::DIAG;
-                if (e < nrow + SL::DIAG)
-                    return wk.slots + (i64)par * G * SL::SIZE + SL::DIAG + (e - nrow);
-                const int f = e - nrow - SL::DIAG, q = f / (WPV + 1), h = f % (WPV + 1);
-                return wk.slots + ((i64)par * G + q) * SL::SIZE + SL::VAL + h;
-            };
-            unsigned spins = 0;
-            bool ok = true;
-            for (int e0 = t; e0 < total && ok; e0 += CW * NTH) {
-                unsigned pend = 0;                        // bit i: word e0 + i*NTH stale
-                static_for<0, CW>([&](auto ic) {
-                    constexpr int i = decltype(ic)::value;
-                    if (e0 + i * NTH < total) pend |= 1u << i;
-                });
-                while (pend) {
-                    unsigned long long wv[CW];
-                    static_for<0, CW>([&](auto ic) {
-                        constexpr int i = decltype(ic)::value;
-                        wv[i] = (pend >> i) & 1u ? cload(src_of(e0 + i * NTH)) : 0ull;
+        static_for<1, NBLK - 1>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            if (j != kk + 1) return;                                  // uniform
+            static_for<0, BB>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const T li = row[NB - BB + i];
+                static_for<0, BB / VN>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value * VN;
+                    const VT u = *reinterpret_cast<const VT*>(&s_rows[i][(j + 1) * BB + q]);
+                    static_for<0, VN>([&](auto uc) {
+                        constexpr int x = decltype(uc)::value;
+                        row[j * BB + q + x] -= li * u[x];
                     });
-                    static_for<0, CW>([&](auto ic) {
-                        constexpr int i = decltype(ic)::value;
-                        if (((pend >> i) & 1u) && (wv[i] & ~0xffffffffull) == tag) {
-                            pend &= ~(1u << i);
-                            const int e = e0 + i * NTH;
-                            const unsigned pay = (unsigned)wv[i];
-                            if (e < nrow) s_cand[e / SL::DIAG][e % SL::DIAG] = pay;
-                            else if (e < nrow + SL::DIAG) s_diag[e - nrow] = pay;
-                            else {
-                                const int f = e - nrow - SL::DIAG, q = f / (WPV + 1),
-                                          h = f % (WPV + 1);
-                                if (h < WPV) s_cval[q][h] = pay; else s_cidx[q] = (int)pay;
-                            }
+                });
+            });
+            // keeps the per-chunk branches apart: merged, they would index
+            // row[] with the runtime chunk number and move it to memory
+            __asm__ volatile("" ::: "memory");
+        });
+    };
+    // Bounded wait bookkeeping: false once the wait has to give up --
+    // another workgroup raised the error flag, or no progress for WAIT_TICKS.
+    auto still_waiting = [&](unsigned& spins, long long& t0) __attribute__((always_inline)) -> bool {
+        if ((++spins & 63) != 0) return true;
+        if (__hip_atomic_load(wk.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+        const long long now = wall_clock64();
+        if (t0 == 0) t0 = now;
+        if (now - t0 > WAIT_TICKS) {
+            __hip_atomic_store(wk.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        return true;
+    };
+
+    const int nblk = (jb + BB - 1) / BB;
+    for (int kb = 0; kb < nblk && alive; ++kb) {
+        const int c0 = kb * BB;
+        const int bcnt = (jb - c0) < BB ? (jb - c0) : BB;
+        int kk = 0;
+        for (; kk < bcnt; ++kk) {
+            const int k = c0 + kk;
+            const int par = k & 1;
+            const unsigned tag32 = (unsigned)(j0 + k + 1);
+            const unsigned long long tag = (unsigned long long)tag32 << 32;
+            // -- wave argmax over the unused rows ---------------------------
+            T v;
+            int vi;
+            lanes_argmax<64>(used ? T(-1) : fabs(row[0]), pos, v, vi);
+            if (lane == 0) { s_wv[par][wave] = v; s_wi[par][wave] = vi; }
+            if (!used && pos == vi) {
+                typedef typename Vec16<T>::type VT;
+                constexpr int VN = Vec16<T>::n;
+                static_for<0, BB / VN>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value * VN;
+                    VT x;
+                    static_for<0, VN>([&](auto uc) { x[decltype(uc)::value] = row[q + decltype(uc)::value]; });
+                    *reinterpret_cast<VT*>(&s_wrow[par][wave][q]) = x;
+                });
+            }
+            __syncthreads();
+            PANEL_TRACE(k, 0);
+            if (wave != 0) deferred_chunk(kk);     // inside the exchange wait
+            if (wave == 0) {
+                // -- this workgroup's candidate, published as tagged words;
+                //    word f of workgroup q lives at cs[f * G + q] ----------
+                unsigned long long* cs = wk.cslots + (i64)par * CS::SIZE * G;
+                const int myi = lane < NW ? s_wi[par][lane] : INT_MAX;
+                T bv;
+                int bi;
+                lanes_argmax<8>(lane < NW ? s_wv[par][lane] : T(-1), myi, bv, bi);
+                const unsigned long long hw = __ballot(lane < NW && myi == bi);
+                const int ww = hw ? (int)__builtin_ctzll(hw) : 0;
+                if (lane < CS::SIZE) {
+                    const int e = lane >= CS::ROW ? lane - CS::ROW : 0;
+                    const unsigned rowpay = half_bits(s_wrow[par][ww][e / WPV], e % WPV);
+                    const unsigned pay = lane < CS::POS ? half_bits(bv, lane)
+                                       : lane == CS::POS ? (unsigned)bi : rowpay;
+                    cstore(cs + (i64)lane * G + w, tag | pay);
+                }
+                PANEL_TRACE(k, 1);
+                deferred_chunk(kk);                // before any slot can arrive
+                // -- receive every slot: lane q holds slot q ----------------
+                T bestv = T(-1);
+                int besti = INT_MAX;
+                unsigned bpay[BW];
+#pragma unroll
+                for (int f = 0; f < BW; ++f) bpay[f] = 0u;
+                bool ok = true;
+                for (int qb = 0; qb < G && ok; qb += 64) {
+                    const int q = qb + lane;
+                    const bool valid = q < G;
+                    const unsigned long long* src = cs + (valid ? q : qb);
+                    unsigned long long wv[CS::SIZE];
+                    unsigned spins = 0;
+                    long long t0 = 0;
+                    while (true) {
+                        static_for<0, CS::SIZE>([&](auto fc) {
+                            constexpr int f = decltype(fc)::value;
+                            wv[f] = cload(src + (i64)f * G);
+                        });
+                        bool have = true;
+                        static_for<0, CS::SIZE>([&](auto fc) {
+                            constexpr int f = decltype(fc)::value;
+                            have &= (unsigned)(wv[f] >> 32) == tag32;
+                        });
+                        if (__ballot(valid && !have) == 0ull) break;
+                        if (!still_waiting(spins, t0)) { ok = false; break; }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    if (ok && valid) {
+                        unsigned pay[CS::SIZE];
+#pragma unroll
+                        for (int f = 0; f < CS::SIZE; ++f) pay[f] = (unsigned)wv[f];
+                        const T qv = from_halves<T>(&pay[CS::VAL]);
+                        const int qi = (int)pay[CS::POS];
+                        if (better(qv, qi, bestv, besti)) {
+                            bestv = qv;
+                            besti = qi;
+#pragma unroll
+                            for (int f = 0; f < BW; ++f) bpay[f] = pay[CS::ROW + f];
+                        }
+                    }
+                }
+                PANEL_TRACE(k, 2);
+                if (ok) {
+                    // -- the global winner (the same in every workgroup) ----
+                    T gv;
+                    int gi;
+                    lanes_argmax<64>(bestv, besti, gv, gi);
+                    const unsigned long long hg = __ballot(besti == gi);
+                    const int wl = hg ? (int)__builtin_ctzll(hg) : 0;
+                    const bool zero = !(gv >= tiny<T>());              // TMV_Underflow
+                    const int p = zero ? k : gi;
+                    // the winner's block values: lane j < BB takes value j
+                    unsigned hv[WPV];
+                    static_for<0, BB>([&](auto jc) {
+                        constexpr int j = decltype(jc)::value;
+#pragma unroll
+                        for (int x = 0; x < WPV; ++x) {
+                            const unsigned u = (unsigned)__builtin_amdgcn_readlane((int)bpay[j * WPV + x], wl);
+                            if (lane == j) hv[x] = u;
                         }
                     });
-                    if (!pend) break;
-                    if (++spins > SPIN_LIMIT ||
-                        ((spins & 255) == 0 &&
-                         __hip_atomic_load(wk.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-                        __hip_atomic_store(wk.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        s_fail = 1;
-                        ok = false;
-                        break;
+                    if (lane < BB) s_pv[lane] = from_halves<T>(hv);
+                    if (lane == 0) {
+                        s_zero = zero ? 1 : 0;
+                        s_p = p;
+                        if (w == 0) {
+                            piv[j0 + k] = j0 + p;
+                            if (zero)
+                                atomicMin(reinterpret_cast<unsigned long long*>(wk.first_zero),
+                                          (unsigned long long)(j0 + k + 1));
+                        }
                     }
-                    __builtin_amdgcn_s_sleep(1);
+                } else if (lane == 0) {
+                    s_fail = 1;
                 }
             }
-        }
-        PANEL_TRACE(k, 3);
-        __syncthreads();
-        PANEL_TRACE(k, 4);
-        if (s_fail) { alive = false; break; }
-        // -- global winner (identical in every workgroup): lane q of wave 0
-        //    holds slab q's candidate, one DPP argmax picks the best --------
-        if (wave == 0) {
-            T bv = T(-1);
-            int bi = INT_MAX;
-            if (lane < G) { bv = from_halves<T>(s_cval[lane]); bi = s_cidx[lane]; }
-            const T myv = bv;
-            const int myi = bi;
-            wave_argmax(bv, bi);
-            // the winning slab: lowest lane holding the winning candidate
-            const unsigned long long hit = __ballot(lane < G && myv == bv && myi == bi);
-            if (lane == 0) {
-                const bool zero = !(bv >= tiny<T>());   // TMV_Underflow
-                s_zero = zero ? 1 : 0;
-                s_p = zero ? k : bi;
-                s_win = hit ? (int)__builtin_ctzll(hit) : 0;
-            }
-        }
-        __syncthreads();
-        if (t < NB) s_prow[t] = from_halves<T>(&s_cand[s_win][t * WPV]);
-        else if (t < 2 * NB) s_orow[t - NB] = from_halves<T>(&s_diag[(t - NB) * WPV]);
-        __syncthreads();
-        PANEL_TRACE(k, 5);
-        const int p = s_p;
-        if (s_zero) {
-            if (own && r >= k) row[0] = T(0);
-            if (w == 0 && t == 0) {
-                piv[j0 + k] = j0 + k;
-                atomicMin(reinterpret_cast<unsigned long long*>(wk.first_zero),
-                          (unsigned long long)(j0 + k + 1));
-            }
-        } else {
-            if (own && p != k && r == p) {
-#pragma unroll
-                for (int c = 0; c < NB; ++c) row[c] = s_orow[c];
-            }
-            if (own && r == k) {
-#pragma unroll
-                for (int c = 0; c < NB; ++c) row[c] = s_prow[c];
-            }
-            if (own && r > k) {
-                // rotated columns 1 .. NB-1-k are the panel columns k+1 .. NB-1
-                if constexpr (sizeof(T) == 4) {
-                    T pr[NB];                   // pivot row, 16-byte LDS loads
-                    static_for<0, NB / VN>([&](auto cc) {
-                        constexpr int c = decltype(cc)::value * VN;
-                        const VT x = *reinterpret_cast<const VT*>(&s_prow[c]);
-                        static_for<0, VN>([&](auto uc) {
-                            pr[c + decltype(uc)::value] = x[decltype(uc)::value];
-                        });
+            __syncthreads();
+            if (s_fail) { alive = false; break; }                    // uniform after the barrier
+            // -- interchange (positions only) and the block's update ---------
+            const int p = s_p;
+            if (s_zero) {
+                // TMV: zero the column from the diagonal down, interchange
+                // with itself; the row at position k is the pivot row.
+                if (!used) row[0] = T(0);
+                if (!used && pos == k) { used = true; mypiv = kk; }
+            } else {
+                if (!used && pos == p) { used = true; mypiv = kk; pos = k; }
+                else if (!used && pos == k) pos = p;
+                if (!used) {
+                    typedef typename Vec16<T>::type VT;
+                    constexpr int VN = Vec16<T>::n;
+                    T pv[BB];
+                    static_for<0, BB / VN>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value * VN;
+                        const VT x = *reinterpret_cast<const VT*>(&s_pv[q]);
+                        static_for<0, VN>([&](auto uc) { pv[q + decltype(uc)::value] = x[decltype(uc)::value]; });
                     });
-                    const T l = row[0] * (T(1) / pr[0]);
+                    const T l = row[0] * (T(1) / pv[0]);
                     row[0] = l;
+                    const int lim = BB - kk;       // block columns after this one
 #pragma unroll
-                    for (int c = 1; c < NB; ++c)
-                        if (c < NB - k) row[c] -= l * pr[c];
-                } else {                        // (registers: the row alone is 128)
-                    const T l = row[0] * (T(1) / s_prow[0]);
-                    row[0] = l;
-#pragma unroll
-                    for (int c = 1; c < NB; ++c)
-                        if (c < NB - k) row[c] -= l * s_prow[c];
+                    for (int c = 1; c < BB; ++c)
+                        if (c < lim) row[c] -= l * pv[c];
                 }
             }
-            if (w == 0 && t == 0) piv[j0 + k] = j0 + p;
+            rotate_in_block();
+            PANEL_TRACE(k, 3);
         }
-        PANEL_TRACE(k, 6);
-        rotate();
-        ++nrot;
-        cv = own ? fabs(row[0]) : T(-1);
+        if (!alive) break;
+        // complete the in-block rotation of a partial block
+        for (; kk < BB; ++kk) rotate_in_block();
+        if (c0 + BB >= jb) break;
+        // -- block end: the BB pivot rows, U12, rank-BB update ---------------
+        if (mypiv >= 0) {
+            s_own[mypiv] = 1;
+            typedef typename Vec16<T>::type VT;
+            constexpr int VN = Vec16<T>::n;
+            static_for<0, NB / VN>([&](auto qc) {
+                constexpr int q = decltype(qc)::value * VN;
+                VT x;
+                static_for<0, VN>([&](auto uc) { x[decltype(uc)::value] = row[q + decltype(uc)::value]; });
+                *reinterpret_cast<VT*>(&s_rows[mypiv][q]) = x;
+            });
+        }
         __syncthreads();
-        PANEL_TRACE(k, 7);
+        {
+            const int bpar = kb & 1;
+            const unsigned long long btag = (unsigned long long)(j0 + c0 + 1) << 32;
+            unsigned long long* bs = wk.bslots + (i64)bpar * BB * RW;
+            for (int e = t; e < BB * RW; e += NTH)
+                if (s_own[e / RW]) cstore(bs + e, btag | s_rw[e]);
+            // words held elsewhere: all of a thread's loads in flight at once
+            constexpr int RPT = (BB * RW + NTH - 1) / NTH;
+            unsigned pend = 0;
+            static_for<0, RPT>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const int e = t + i * NTH;
+                if (e < BB * RW && !s_own[e / RW]) pend |= 1u << i;
+            });
+            unsigned spins = 0;
+            long long t0 = 0;
+            while (pend) {
+                unsigned long long x[RPT];
+                static_for<0, RPT>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    const int e = t + i * NTH;
+                    x[i] = cload(bs + (e < BB * RW ? e : 0));
+                });
+                static_for<0, RPT>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    if (((pend >> i) & 1u) && (x[i] & ~0xffffffffull) == btag) {
+                        pend &= ~(1u << i);
+                        s_rw[t + i * NTH] = (unsigned)x[i];
+                    }
+                });
+                if (!pend) break;
+                if (!still_waiting(spins, t0)) { s_fail = 1; break; }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __syncthreads();
+        if (s_fail) { alive = false; break; }
+        if (t < BB) s_own[t] = 0;
+        // U12 = L11^-1 A12 in place: thread c solves column c of the rows
+        // (L11(i, j) = s_rows[i][j], j < i, unit diagonal).
+        if (t >= BB && t < NB - c0) {
+            T u[BB];
+            static_for<0, BB>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                T x = s_rows[i][t];
+                static_for<0, i>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    x -= s_rows[i][j] * u[j];
+                });
+                u[i] = x;
+                s_rows[i][t] = x;
+            });
+        }
+        __syncthreads();
+        dfr = false;
+        if (mypiv >= 0) {
+            // this block's pivot rows take their U12 row
+            static_for<1, NBLK>([&](auto chc) {
+                constexpr int ch = decltype(chc)::value;
+                if (ch * BB < NB - c0) {
+#pragma unroll
+                    for (int q = 0; q < BB; ++q) row[ch * BB + q] = s_rows[mypiv][ch * BB + q];
+                }
+            });
+            mypiv = -1;
+        } else if (!used) {
+            // rank-BB update of the next block's columns now; the columns
+            // beyond it follow chunk by chunk during the next block's steps
+            typedef typename Vec16<T>::type VT;
+            constexpr int VN = Vec16<T>::n;
+            static_for<0, BB>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const T li = row[i];
+                static_for<0, BB / VN>([&](auto qc) {
+                    constexpr int q = BB + decltype(qc)::value * VN;
+                    const VT u = *reinterpret_cast<const VT*>(&s_rows[i][q]);
+                    static_for<0, VN>([&](auto uc) {
+                        constexpr int x = decltype(uc)::value;
+                        row[q + x] -= li * u[x];
+                    });
+                });
+            });
+            dfr = true;
+        }
+        rotate_blocks();
+        nchunk = nblk - kb - 2 > 0 ? nblk - kb - 2 : 0;
+        // s_rows (U12) is read by the deferred chunks of the next block and
+        // rewritten only at its end, after every chunk has been applied
     }
-    // Restore column order (jb < NB leaves the rows part-way round).
-    for (int q = nrot % NB; q != 0 && q < NB; ++q) rotate();
-
+    // Restore column order (the rotation is by whole blocks).
+    while ((nrot * BB) % NB != 0) rotate_blocks();
+    if (!alive || !own) return;
+    T* dst = A + (j0 + pos) + j0 * lda;
 #pragma unroll
     for (int c = 0; c < NB; ++c)
-        if (own && c < jb) gp[(i64)c * lda] = row[c];
+        if (c < jb) dst[(i64)c * lda] = row[c];
 }
 
 // Net effect of interchanges piv[j0 .. j0+jb) (absolute rows, each >= its
@@ -511,11 +691,14 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
     *info = 0;
     if (n == 0) return TMVX_OK;
     if (!A || !piv) return TMVX_ERR_ARG;
-    constexpr int NB = PANEL_NB, NTH = PANEL_THREADS;
+    constexpr int NB = PANEL_NB, BB = PANEL_BB, NTH = PANEL_THREADS;
+    typedef ColSlot<T, BB> CS;
     const int Gmax = (int)((n + NTH - 1) / NTH);
-    if (Gmax > PANEL_GMAX) return TMVX_ERR_UNSUPPORTED;   // n > 32768
+    if (Gmax > PANEL_GMAX) return TMVX_ERR_UNSUPPORTED;   // n > 262144
 
-    const i64 nslot = 2 * (i64)Gmax * SlotLayout<T, NB>::SIZE;
+    const i64 ncslot = 2 * (i64)Gmax * CS::SIZE;
+    const i64 nbslot = 2 * (i64)BB * NB * CS::WPV;
+    const i64 nslot = ncslot + nbslot;
     const size_t szslots = sizeof(unsigned long long) * (size_t)nslot;
     const bool tslu = sizeof(T) == 4 && tournament != 0;
     const size_t szts = tslu ? sizeof(int) * (size_t)tslu_ws_ints(n) : 0;
@@ -523,7 +706,8 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
     char* ws = nullptr;
     if (hipMallocAsync((void**)&ws, total, s) != hipSuccess) return TMVX_ERR_ALLOC;
     PanelWork wk;
-    wk.slots = reinterpret_cast<unsigned long long*>(ws);
+    wk.cslots = reinterpret_cast<unsigned long long*>(ws);
+    wk.bslots = wk.cslots + ncslot;
     char* tail = ws + szslots;
     wk.err = reinterpret_cast<int*>(tail);
     wk.first_zero = reinterpret_cast<i64*>(tail + 64);
@@ -547,7 +731,7 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
     {
         const i64 cnt = n > nslot ? n : nslot;
         hipLaunchKernelGGL(getrf_init_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256),
-                           0, s, wk.slots, nslot, wk.err, wk.first_zero, n, perm);
+                           0, s, wk.cslots, nslot, wk.err, wk.first_zero, n, perm);
         rc = hip_status(hipGetLastError());
     }
     // Interchanges of panel [j0, j0+jb) on columns [a0,a1) u [b0,b1).
@@ -597,6 +781,7 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
     // panel's interchanges outside it wait for that GEMM (it reads the L
     // and U blocks the interchanges would move).
     constexpr i64 NBO = 4 * NB;
+    SideSet* ss = nullptr;                 // taken from the pool on first need
     hipStream_t side = nullptr;
     hipEvent_t ev_main = nullptr, ev_side = nullptr;
     bool side_pending = false;
@@ -615,7 +800,7 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
             }
             if (!tslu) {
                 const int G = (int)((n - j0 + NTH - 1) / NTH);
-                hipLaunchKernelGGL((panel_kernel<T, NB, NTH>), dim3(G), dim3(NTH), 0, s,
+                hipLaunchKernelGGL((panel_kernel<T, NB, BB, NTH>), dim3(G), dim3(NTH), 0, s,
                                    A, lda, n, j0, jb, piv, wk);
                 if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
             }
@@ -654,12 +839,10 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
         if (rc != TMVX_OK || JE2 >= n) continue;
         // the rest of the trailing matrix on the side stream
         if (!side) {
-            if (hipStreamCreateWithFlags(&side, hipStreamNonBlocking) != hipSuccess ||
-                hipEventCreateWithFlags(&ev_main, hipEventDisableTiming) != hipSuccess ||
-                hipEventCreateWithFlags(&ev_side, hipEventDisableTiming) != hipSuccess) {
-                rc = TMVX_ERR_HIP;
-                break;
-            }
+            if ((rc = side_acquire(&ss)) != TMVX_OK) break;
+            side = ss->stream;
+            ev_main = ss->ev_main;
+            ev_side = ss->ev_side;
         }
         if (hipEventRecord(ev_main, s) != hipSuccess ||
             hipStreamWaitEvent(side, ev_main, 0) != hipSuccess) {
@@ -691,11 +874,7 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
         rc = TMVX_ERR_HIP;
     hipFreeAsync(ws, s);
     const bool sync_ok = hipStreamSynchronize(s) == hipSuccess;
-    if (side) {
-        hipEventDestroy(ev_main);
-        hipEventDestroy(ev_side);
-        hipStreamDestroy(side);
-    }
+    side_release(ss);                      // its work is done: s waited for it
     if (!sync_ok) return TMVX_ERR_HIP;
     if (rc != TMVX_OK) return rc;
     if (herr) return TMVX_ERR_TIMEOUT;
@@ -794,9 +973,11 @@ template int getrf_impl<float>(i64, float*, i64, i64*, i64*, i64*, hipStream_t, 
 } // namespace tmvx
 
 #ifdef TMVX_PANEL_TRACE
-extern "C" int tmvx_panel_trace_set(unsigned long long* buf)
+extern "C" int tmvx_panel_trace_set(unsigned long long* buf, long long j0)
 {
-    return hipMemcpyToSymbol(HIP_SYMBOL(tmvx::g_panel_trace), &buf, sizeof(buf)) == hipSuccess
+    return hipMemcpyToSymbol(HIP_SYMBOL(tmvx::g_panel_trace), &buf, sizeof(buf)) == hipSuccess &&
+                   hipMemcpyToSymbol(HIP_SYMBOL(tmvx::g_panel_trace_j0), &j0, sizeof(j0)) ==
+                       hipSuccess
                ? 0 : -3;
 }
 #endif

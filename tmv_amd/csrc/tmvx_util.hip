@@ -6,8 +6,52 @@
 // The same definition is in tmv_amd/rng.py; uniform values are bit-identical
 // (the affine map is computed without contraction).
 #include "tmvx_common.h"
+#include "tmvx_lu.h"
+#include <mutex>
+#include <vector>
 
 namespace tmvx {
+
+// Side streams of the look-ahead factorisation: a per-device free list of
+// (stream, two events) sets, created on first need and kept until the
+// process ends, so a call creates nothing once the pool holds as many sets
+// as calls run at once.  Default stream priority.
+namespace {
+std::mutex g_side_mu;
+std::vector<std::vector<SideSet*>> g_side_free;   // [device]
+}
+
+int side_acquire(SideSet** out)
+{
+    *out = nullptr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return TMVX_ERR_HIP;
+    {
+        std::lock_guard<std::mutex> lk(g_side_mu);
+        if ((int)g_side_free.size() <= dev) g_side_free.resize(dev + 1);
+        auto& fl = g_side_free[dev];
+        if (!fl.empty()) { *out = fl.back(); fl.pop_back(); return TMVX_OK; }
+    }
+    SideSet* ss = new SideSet;
+    ss->device = dev;
+    if (hipStreamCreateWithFlags(&ss->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ss->ev_main, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ss->ev_side, hipEventDisableTiming) != hipSuccess) {
+        if (ss->ev_main) hipEventDestroy(ss->ev_main);
+        if (ss->stream) hipStreamDestroy(ss->stream);
+        delete ss;
+        return TMVX_ERR_HIP;
+    }
+    *out = ss;
+    return TMVX_OK;
+}
+
+void side_release(SideSet* ss)
+{
+    if (!ss) return;
+    std::lock_guard<std::mutex> lk(g_side_mu);
+    g_side_free[ss->device].push_back(ss);
+}
 
 __host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x)
 {
