@@ -313,39 +313,48 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb, i64* __restrict_
 #pragma unroll
                 for (int f = 0; f < BW; ++f) bpay[f] = 0u;
                 bool ok = true;
-                for (int qb = 0; qb < G && ok; qb += 64) {
-                    const int q = qb + lane;
-                    const bool valid = q < G;
-                    const unsigned long long* src = cs + (valid ? q : qb);
-                    unsigned long long wv[CS::SIZE];
-                    unsigned spins = 0;
-                    long long t0 = 0;
-                    while (true) {
-                        static_for<0, CS::SIZE>([&](auto fc) {
-                            constexpr int f = decltype(fc)::value;
-                            wv[f] = cload(src + (i64)f * G);
-                        });
-                        bool have = true;
-                        static_for<0, CS::SIZE>([&](auto fc) {
-                            constexpr int f = decltype(fc)::value;
-                            have &= (unsigned)(wv[f] >> 32) == tag32;
-                        });
-                        if (__ballot(valid && !have) == 0ull) break;
-                        if (!still_waiting(spins, t0)) { ok = false; break; }
-                        __builtin_amdgcn_s_sleep(1);
+                auto take = [&](const unsigned long long (&wv)[CS::SIZE], bool valid)
+                    __attribute__((always_inline)) {
+                    if (!valid) return;
+                    unsigned pay[CS::SIZE];
+#pragma unroll
+                    for (int f = 0; f < CS::SIZE; ++f) pay[f] = (unsigned)wv[f];
+                    const T qv = from_halves<T>(&pay[CS::VAL]);
+                    const int qi = (int)pay[CS::POS];
+                    if (better(qv, qi, bestv, besti)) {
+                        bestv = qv;
+                        besti = qi;
+#pragma unroll
+                        for (int f = 0; f < BW; ++f) bpay[f] = pay[CS::ROW + f];
                     }
-                    if (ok && valid) {
-                        unsigned pay[CS::SIZE];
-#pragma unroll
-                        for (int f = 0; f < CS::SIZE; ++f) pay[f] = (unsigned)wv[f];
-                        const T qv = from_halves<T>(&pay[CS::VAL]);
-                        const int qi = (int)pay[CS::POS];
-                        if (better(qv, qi, bestv, besti)) {
-                            bestv = qv;
-                            besti = qi;
-#pragma unroll
-                            for (int f = 0; f < BW; ++f) bpay[f] = pay[CS::ROW + f];
+                };
+                auto complete = [&](const unsigned long long (&wv)[CS::SIZE], bool valid)
+                    __attribute__((always_inline)) -> bool {
+                    bool have = true;
+                    static_for<0, CS::SIZE>([&](auto fc) {
+                        constexpr int f = decltype(fc)::value;
+                        have &= (unsigned)(wv[f] >> 32) == tag32;
+                    });
+                    return __ballot(valid && !have) == 0ull;
+                };
+                {
+                    for (int qb = 0; qb < G && ok; qb += 64) {
+                        const int q = qb + lane;
+                        const bool valid = q < G;
+                        const unsigned long long* src = cs + (valid ? q : qb);
+                        unsigned long long wv[CS::SIZE];
+                        unsigned spins = 0;
+                        long long t0 = 0;
+                        while (true) {
+                            static_for<0, CS::SIZE>([&](auto fc) {
+                                constexpr int f = decltype(fc)::value;
+                                wv[f] = cload(src + (i64)f * G);
+                            });
+                            if (complete(wv, valid)) break;
+                            if (!still_waiting(spins, t0)) { ok = false; break; }
+                            __builtin_amdgcn_s_sleep(1);
                         }
+                        if (ok) take(wv, valid);
                     }
                 }
                 PANEL_TRACE(k, 2);
