@@ -263,21 +263,36 @@ int tmvx_laswp_f32(int forward, int64_t ncols, float* A, int64_t lda,
  * arithmetic of the trailing updates: FP32, or FP16 / BF16 with FP32
  * accumulation and the U block row scaled by a power of two per outer
  * panel), then for each of the nrhs columns
- *   r = b - A x  (FP64, on the GPU),  solve LU d = P r / s in FP32 with
- *   s = max|r|,  x += s d  (FP64)
+ *   r = b - A x  (FP64, on the GPU),  then the correction d:
+ *     classical: solve LU d = P r / s in FP32 with s = max|r|, d *= s;
+ *     GMRES-IR:  A d = r by GMRES in FP64 preconditioned by the factors
+ *                (see tmvx_refine below);
+ *   x += d  (FP64)
  * until the normwise backward error ||r||_inf / (||A||_inf ||x||_inf +
  * ||b||_inf) of every column is <= tol, or max_iter refinement steps.
  * X (n x nrhs, FP64) receives the solution.  Synchronises.  Returns
- * TMVX_ERR_NOT_CONVERGED (with X holding the best iterate) when the target
- * is not met.
+ * TMVX_ERR_NOT_CONVERGED (with X holding the last iterate and
+ * info->backward_error its backward error) when the target is not met.
+ * lowprec may carry a tmvx_refine flag.
  */
 enum tmvx_lowprec { TMVX_LOWPREC_F32 = 0, TMVX_LOWPREC_F16 = 1, TMVX_LOWPREC_BF16 = 2 };
+
+/*
+ * Refinement method, or-ed into lowprec.  Default: classical refinement
+ * after an FP32 factorisation; GMRES-IR after FP16 / BF16 trailing updates.
+ * GMRES-IR (Carson & Higham's three-precision refinement) solves the
+ * correction equation A d = r by GMRES in FP64, left-preconditioned by the
+ * low-precision factors (restart 50, relative inner tolerance 1e-8, one
+ * right-hand side at a time); it converges where kappa * u_low >~ 1 makes
+ * classical refinement stall or diverge.
+ */
+enum tmvx_refine { TMVX_REFINE_GMRES = 16, TMVX_REFINE_CLASSIC = 32 };
 
 typedef struct tmvx_ir_info {
     int iterations;            /* refinement steps taken                  */
     int converged;             /* 1 when every column met tol             */
     int factor_info;           /* first zero pivot of the low-precision LU */
-    int reserved;
+    int inner_iterations;      /* GMRES steps over all refinement steps   */
     double backward_error;     /* worst column, final iterate             */
     double factor_ms;          /* device time of the factorisation        */
     double refine_ms;          /* device time of all refinement steps     */

@@ -1,13 +1,18 @@
 """FP16 / BF16 trailing updates (tmvx_gemm_f32_lowp) and the mixed-precision
-solver with FP16 / BF16 factorisations, through the C ABI.
+solver with FP16 / BF16 factorisations (GMRES-IR by default, classical
+refinement on request), through the C ABI.
 
 Bar for the GEMM: elementwise |C - C_ref| <= 2 (2 u16 + k u32) (|A||B|)_ij
 (+ beta term) against a long-double product of the unrounded FP32 inputs,
 u16 the unit roundoff of the 16-bit format (FP16 2^-11, BF16 2^-8): each
 operand is rounded once, products of two 16-bit values are exact in FP32,
 sums accumulate in FP32.  Scaling keeps FP16 accurate for B far outside its
-range.  For the solver: backward error <= 1e-12 when it reports
-convergence (kappa = 1e2), and a clean not-converged report otherwise."""
+range.  For the solver: backward error <= 1e-12 (recomputed on the host in
+long double) with GMRES-IR for FP16 up to kappa = 1e4 (n up to 4096) and
+BF16 at kappa = 1e2; classical refinement converges at kappa = 1e2.  Where
+the 16-bit factors are too weak a preconditioner for GMRES(50) (BF16 at
+kappa = 1e4, FP16 at 1e6 on these log-spaced spectra) the solver reports
+not-converged instead of returning an unconverged iterate."""
 import numpy as np
 import pytest
 import torch
@@ -66,34 +71,115 @@ def conditioned(n, kappa, seed):
     return (U * s) @ V.T
 
 
-@pytest.mark.parametrize("lp", [tmv_amd.LOWPREC_F16, tmv_amd.LOWPREC_BF16])
-@pytest.mark.parametrize("n", [600, 2048])
-def test_mixed_lowp_converges(dev, lp, n):
-    print(f"mixed {NAME[lp]} n={n} kappa=1e2", flush=True)
-    A = conditioned(n, 1e2, SEED + n)
-    b = np.random.default_rng(n).standard_normal(n)
-    dA = tmv_amd.colmajor(torch.from_numpy(A).cuda())
+def conditioned_gpu(n, kappa, seed):
+    """The same construction on the GPU (torch QR), column-major FP64."""
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    U, _ = torch.linalg.qr(torch.randn(n, n, generator=g, dtype=torch.float64).cuda())
+    V, _ = torch.linalg.qr(torch.randn(n, n, generator=g, dtype=torch.float64).cuda())
+    s = torch.logspace(0, -np.log10(kappa), n, dtype=torch.float64, device="cuda")
+    return tmv_amd.colmajor((U * s[None, :]) @ V.t().contiguous())
+
+
+def solve_check(A, dA, b, lp, max_iter=20):
     db = torch.from_numpy(b).cuda()
-    x, info = tmv_amd.gesv_mixed(dA, db, tol=1e-12, max_iter=60, lowprec=lp)
+    x, info = tmv_amd.gesv_mixed(dA, db, tol=1e-12, max_iter=max_iter, lowprec=lp)
     torch.cuda.synchronize()
     assert info["converged"] == 1 and info["status"] == tmv_amd.OK, info
     assert info["backward_error"] <= 1e-12
     assert backward_error(A, x.cpu().numpy(), b) <= 1e-12
+    return info
+
+
+F16, BF16 = tmv_amd.LOWPREC_F16, tmv_amd.LOWPREC_BF16
+
+
+@pytest.mark.parametrize("lp,n,kappa", [(F16, 600, 1e2), (F16, 2048, 1e2), (F16, 600, 1e4),
+                                        (F16, 1500, 1e4), (BF16, 600, 1e2), (BF16, 2048, 1e2)])
+def test_mixed_lowp_gmres_converges(dev, lp, n, kappa):
+    """GMRES-IR (the default after 16-bit trailing updates)."""
+    print(f"mixed gmres {NAME[lp]} n={n} kappa={kappa:g}", flush=True)
+    A = conditioned(n, kappa, SEED + n + int(np.log10(kappa)))
+    b = np.random.default_rng(n).standard_normal(n)
+    info = solve_check(A, tmv_amd.colmajor(torch.from_numpy(A).cuda()), b, lp)
+    assert info["inner_iterations"] >= info["iterations"] > 0
+
+
+def test_mixed_lowp_gmres_n4096_kappa1e4(dev):
+    """n = 4096, kappa = 1e4: FP16 trailing updates with GMRES-IR reach a
+    backward error <= 1e-12."""
+    n = 4096
+    print(f"mixed gmres f16 n={n} kappa=1e4", flush=True)
+    dA = conditioned_gpu(n, 1e4, SEED + 4096)
+    A = dA.cpu().numpy()
+    b = np.random.default_rng(4096).standard_normal(n)
+    solve_check(A, dA, b, F16)
+
+
+@pytest.mark.parametrize("lp,kappa", [(BF16, 1e4), (F16, 1e6)])
+def test_mixed_lowp_gmres_stall_reported(dev, lp, kappa):
+    """Beyond what GMRES(50) can recover from the 16-bit factors (BF16 at
+    kappa = 1e4, FP16 at 1e6, n = 600): a converged status only with a
+    backward error <= 1e-12, otherwise not-converged, never a wrong answer
+    reported as converged."""
+    n = 600
+    A = conditioned(n, kappa, SEED + n + int(np.log10(kappa)))
+    b = np.random.default_rng(n).standard_normal(n)
+    dA = tmv_amd.colmajor(torch.from_numpy(A).cuda())
+    x, info = tmv_amd.gesv_mixed(dA, torch.from_numpy(b).cuda(), tol=1e-12, max_iter=3,
+                                 lowprec=lp, check=False)
+    torch.cuda.synchronize()
+    be = backward_error(A, x.cpu().numpy(), b)
+    if info["status"] == tmv_amd.OK:
+        assert info["converged"] == 1 and be <= 1e-12
+    else:
+        assert info["status"] == tmv_amd.ERR_NOT_CONVERGED and info["converged"] == 0
+        assert abs(info["backward_error"] - be) <= 1e-3 * be   # the reported error is the iterate's
+
+
+def test_mixed_f32_gmres_kappa1e6(dev):
+    """GMRES-IR on request after an FP32 factorisation, kappa = 1e6."""
+    n = 1000
+    A = conditioned(n, 1e6, SEED + 31)
+    b = np.random.default_rng(31).standard_normal(n)
+    info = solve_check(A, tmv_amd.colmajor(torch.from_numpy(A).cuda()), b,
+                       tmv_amd.LOWPREC_F32 | tmv_amd.REFINE_GMRES)
+    assert info["inner_iterations"] > 0
+
+
+@pytest.mark.parametrize("lp", [tmv_amd.LOWPREC_F16, tmv_amd.LOWPREC_BF16])
+def test_mixed_lowp_classic_kappa1e2(dev, lp):
+    """Classical refinement on request after 16-bit updates (kappa = 1e2)."""
+    n = 600
+    A = conditioned(n, 1e2, SEED + n)
+    b = np.random.default_rng(n).standard_normal(n)
+    info = solve_check(A, tmv_amd.colmajor(torch.from_numpy(A).cuda()), b,
+                       lp | tmv_amd.REFINE_CLASSIC, max_iter=60)
+    assert info["inner_iterations"] == 0
 
 
 def test_mixed_lowp_reports_not_converged(dev):
     """kappa = 1e6 is beyond what BF16 trailing updates plus classical
-    refinement can correct: the solver says so instead of returning a
-    wrong answer as converged."""
+    refinement can correct in 5 steps: the solver says so instead of
+    returning a wrong answer as converged."""
     n = 1024
     A = conditioned(n, 1e6, SEED + 9)
     b = np.random.default_rng(9).standard_normal(n)
     dA = tmv_amd.colmajor(torch.from_numpy(A).cuda())
     db = torch.from_numpy(b).cuda()
-    x, info = tmv_amd.gesv_mixed(dA, db, tol=1e-12, max_iter=5, lowprec=tmv_amd.LOWPREC_BF16,
+    x, info = tmv_amd.gesv_mixed(dA, db, tol=1e-12, max_iter=5,
+                                 lowprec=tmv_amd.LOWPREC_BF16 | tmv_amd.REFINE_CLASSIC,
                                  check=False)
     torch.cuda.synchronize()
     if info["status"] == tmv_amd.OK:
         assert backward_error(A, x.cpu().numpy(), b) <= 1e-12
     else:
         assert info["status"] == tmv_amd.ERR_NOT_CONVERGED and info["converged"] == 0
+
+
+def test_mixed_refine_flags_rejected(dev):
+    """Both refinement flags at once, or an unknown flag: unsupported."""
+    A = tmv_amd.colmajor(torch.eye(8, dtype=torch.float64).cuda())
+    b = torch.ones(8, dtype=torch.float64).cuda()
+    for lp in (tmv_amd.REFINE_GMRES | tmv_amd.REFINE_CLASSIC, 64, 3):
+        x, info = tmv_amd.gesv_mixed(A, b, lowprec=lp, check=False)
+        assert info["status"] == tmv_amd.ERR_UNSUPPORTED

@@ -521,17 +521,19 @@ def laswp(A, piv, k1=0, k2=None, forward=True, stream=None):
 class IrInfo(ctypes.Structure):
     """struct tmvx_ir_info (include/tmvx.h)."""
     _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int),
-                ("factor_info", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("factor_info", ctypes.c_int), ("inner_iterations", ctypes.c_int),
                 ("backward_error", ctypes.c_double), ("factor_ms", ctypes.c_double),
                 ("refine_ms", ctypes.c_double)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 LOWPREC_F32 = 0
 LOWPREC_F16 = 1
 LOWPREC_BF16 = 2
+REFINE_GMRES = 16      # or-ed into lowprec: GMRES-IR (default for F16 / BF16)
+REFINE_CLASSIC = 32    # or-ed into lowprec: classical refinement (default for F32)
 
 
 def gesv_mixed(A, B, X=None, tol=1e-12, max_iter=30, lowprec=LOWPREC_F32, check=True,

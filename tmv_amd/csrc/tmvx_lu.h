@@ -48,8 +48,10 @@ int trsm_impl(int upper, int trans, int unit, i64 n, i64 nrhs, const T* A, i64 l
 // Single-launch-per-4-RHS blocked triangular solve (tmvx_trsv.hip):
 // op(A) X = B with B rows gathered through in_perm and X rows scattered
 // through out_perm.  X may alias B only when both permutations are null.
-template <typename T>
-int trsv_multi(int upper, int trans, int unit, i64 n, i64 nrhs, const T* A, i64 lda,
+// TA is A's storage type, T the arithmetic and right-hand side type
+// (instantiated for <double, double>, <float, float>, <double, float>).
+template <typename T, typename TA = T>
+int trsv_multi(int upper, int trans, int unit, i64 n, i64 nrhs, const TA* A, i64 lda,
                const T* B, i64 ldb, const i64* in_perm, T* X, i64 ldx, const i64* out_perm,
                int* d_singular, hipStream_t s);
 

@@ -93,10 +93,13 @@ struct SvWork {
 // added in sub-wave order through LDS, so results are deterministic, and
 // the leading sub-wave solves the diagonal block (loaded first of all, off
 // the dependency chain) and publishes x_b.
-template <typename T>
+//
+// TA is the storage type of A (the arithmetic is T's): float factors with
+// double right-hand sides give FP64 solves with FP32-stored factors.
+template <typename T, typename TA>
 __global__ void __launch_bounds__(256)
 trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
-                       const T* __restrict__ A, i64 lda,
+                       const TA* __restrict__ A, i64 lda,
                        const T* __restrict__ Bin, i64 ldb, const i64* __restrict__ in_perm,
                        T* __restrict__ Xout, i64 ldx, const i64* __restrict__ out_perm,
                        u64* __restrict__ xpub, SvWork wk)
@@ -120,20 +123,20 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
     const int bs = (int)((n - b0) < NB ? (n - b0) : NB);
     const bool row_ok = lane < bs;
     // op(A)(b0 + lane, j) = arow[j * astr]
-    const T* arow = trans ? A + (b0 + lane) * lda : A + (b0 + lane);
+    const TA* arow = trans ? A + (b0 + lane) * lda : A + (b0 + lane);
     const i64 astr = trans ? 1 : lda;
     // op(A)(b0 + lane, c0 + k) for k < cs (0 beyond, and on rows past n)
     auto load_block = [&](T (&blk_)[NB], i64 c0, int cs) __attribute__((always_inline)) {
-        const T* ap = arow + c0 * astr;
+        const TA* ap = arow + c0 * astr;
         if (row_ok && cs == NB) {
             static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
                 constexpr int k = decltype(kc)::value;
-                blk_[k] = ap[k * astr];
+                blk_[k] = (T)ap[k * astr];
             });
         } else {
             static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
                 constexpr int k = decltype(kc)::value;
-                blk_[k] = (row_ok && k < cs) ? ap[k * astr] : T(0);
+                blk_[k] = (row_ok && k < cs) ? (T)ap[k * astr] : T(0);
             });
         }
     };
@@ -144,7 +147,7 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
     if (lead) load_block(drow, b0, bs);
     T rdiag = T(1);
     if (lead && !unit && row_ok) {
-        const T d = arow[(b0 + lane) * astr];
+        const T d = (T)arow[(b0 + lane) * astr];
         rdiag = d == T(0) ? T(0) : T(1) / d;
     }
     T acc = T(0);
@@ -261,8 +264,8 @@ __global__ void sv_init_kernel(unsigned* ticket, u64* words, i64 nwords, int* er
 // op(A) X = B with X and B possibly aliased only when out_perm == in_perm
 // == nullptr and ldx == ldb (each block reads its B rows before writing its
 // X rows, and no other block reads them).
-template <typename T>
-int trsv_multi(int upper, int trans, int unit, i64 n, i64 nrhs, const T* A, i64 lda,
+template <typename T, typename TA>
+int trsv_multi(int upper, int trans, int unit, i64 n, i64 nrhs, const TA* A, i64 lda,
                const T* B, i64 ldb, const i64* in_perm, T* X, i64 ldx, const i64* out_perm,
                int* d_singular, hipStream_t s)
 {
@@ -289,7 +292,7 @@ int trsv_multi(int upper, int trans, int unit, i64 n, i64 nrhs, const T* A, i64 
                            dim3(256), 0, s, wk.ticket, xpub, nz, wk.err);
         if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
         wk.epoch = epoch;
-        hipLaunchKernelGGL(trsv_persistent_kernel<T>, dim3((unsigned)nblk), dim3(256), 0, s,
+        hipLaunchKernelGGL((trsv_persistent_kernel<T, TA>), dim3((unsigned)nblk), dim3(256), 0, s,
                            lower, trans, unit, n, nr, A, lda, B + j0 * ldb, ldb, in_perm,
                            X + j0 * ldx, ldx, out_perm, xpub, wk);
         rc = hip_status(hipGetLastError());
@@ -299,10 +302,14 @@ int trsv_multi(int upper, int trans, int unit, i64 n, i64 nrhs, const T* A, i64 
     return rc;
 }
 
-template int trsv_multi<double>(int, int, int, i64, i64, const double*, i64, const double*, i64,
-                                const i64*, double*, i64, const i64*, int*, hipStream_t);
-template int trsv_multi<float>(int, int, int, i64, i64, const float*, i64, const float*, i64,
-                               const i64*, float*, i64, const i64*, int*, hipStream_t);
+template int trsv_multi<double, double>(int, int, int, i64, i64, const double*, i64,
+                                        const double*, i64, const i64*, double*, i64, const i64*,
+                                        int*, hipStream_t);
+template int trsv_multi<float, float>(int, int, int, i64, i64, const float*, i64, const float*,
+                                      i64, const i64*, float*, i64, const i64*, int*, hipStream_t);
+template int trsv_multi<double, float>(int, int, int, i64, i64, const float*, i64,
+                                       const double*, i64, const i64*, double*, i64, const i64*,
+                                       int*, hipStream_t);
 
 } // namespace tmvx
 
