@@ -85,6 +85,22 @@ int tmvx_getrf_f32(int64_t n, float* A, int64_t lda, int64_t* piv,
                    int64_t* info, void* stream);
 
 /*
+ * tmvx_getrf_f64 that allocates nothing: no counterpart in TMV; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same factorisation
+ * (same kernels in the same order: identical pivots and factors) on
+ * `stream` alone, asynchronous: *d_info (DEVICE int64) receives the 1-based
+ * first zero pivot, 0, or -1 when a bounded device-side wait expired; the
+ * call neither synchronises nor copies to the host.  Its scratch (panel
+ * exchange slots and flags) is a static 64 KiB array per GPU, cleared by
+ * each call: n <= 65536 (larger n returns TMVX_ERR_UNSUPPORTED).  Calls
+ * that overlap on one GPU (from several streams or threads) race on that
+ * array and must be serialised by the caller.
+ */
+int tmvx_getrf_f64_noalloc(int64_t n, double* A, int64_t lda, int64_t* piv,
+                           int64_t* d_info, void* stream);
+
+/*
  * Solve with the factors of tmvx_getrf_*: trans == 0 solves A X = B
  * (permute B's rows by piv, then unit-lower L, then U: TMV LU_LDivEq,
  * src/TMV_LUDiv.cpp:185-226); trans != 0 solves A^T X = B (TMV LU_RDivEq on

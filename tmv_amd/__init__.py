@@ -44,6 +44,7 @@ _SIGS = {
     "tmvx_gemm_f64": (_int, [_int, _int, _i64, _i64, _i64, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64, _vp]),
     "tmvx_gemm_f32": (_int, [_int, _int, _i64, _i64, _i64, _f, _vp, _i64, _vp, _i64, _f, _vp, _i64, _vp]),
     "tmvx_getrf_f64": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
+    "tmvx_getrf_f64_noalloc": (_int, [_i64, _vp, _i64, _vp, _vp, _vp]),
     "tmvx_getrf_f32": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
     "tmvx_getrs_f64": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _int, _vp]),
     "tmvx_getrs_f32": (_int, [_int, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _int, _vp]),

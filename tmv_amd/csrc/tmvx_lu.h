@@ -12,9 +12,22 @@ namespace tmvx {
 // on FP16 / BF16 MFMA with per-block-row scaling (gemm_lowp_launch).
 // tournament != 0 (FP32 only): tournament pivoting (tmvx_tslu.hip) instead
 // of partial pivoting -- for the mixed-precision solver, not for TMV parity.
+//
+// With na != nullptr (no allocation): the workspace is na->ws (at most
+// na->bytes; larger n is TMVX_ERR_UNSUPPORTED), the look-ahead's side work
+// runs on s itself (same kernels, same order, so the same results), and
+// nothing is synchronised or copied to the host: na->d_info (device)
+// receives the info, or -1 when a bounded wait expired.  FP32/FP64 partial
+// pivoting only.
+struct GetrfNoAlloc {
+    void* ws;
+    size_t bytes;
+    i64* d_info;
+};
 template <typename T>
 int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s,
-               int lowprec = TMVX_LOWPREC_F32, int tournament = 0);
+               int lowprec = TMVX_LOWPREC_F32, int tournament = 0,
+               const GetrfNoAlloc* na = nullptr);
 
 // Tournament pivoting (tmvx_tslu.hip): panel [j0, j0+jb) factored in place
 // on its own columns (selection tree, interchange list piv[j0..j0+jb), the

@@ -35,6 +35,7 @@
 #include "tmvx_pivot.h"
 #include <cfloat>
 #include <climits>
+#include <mutex>
 
 namespace tmvx {
 
@@ -678,6 +679,14 @@ __global__ void getrf_init_kernel(unsigned long long* tags, i64 ntags, int* err,
     if (perm != nullptr && i < n) perm[i] = i;
 }
 
+// info of the no-allocation factorisation, on the device: -1 when a bounded
+// wait expired, else the 1-based first zero pivot or 0.
+__global__ void getrf_finish_kernel(const int* err, const i64* first_zero, i64 n, i64* d_info)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *d_info = (err && *err) ? -1 : ((first_zero && *first_zero <= n) ? *first_zero : 0);
+}
+
 __global__ void zero_kernel(unsigned char* p, size_t bytes)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -694,12 +703,18 @@ int zero_bytes(void* p, size_t bytes, hipStream_t s)
 
 template <typename T>
 int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s, int lowprec,
-               int tournament)
+               int tournament, const GetrfNoAlloc* na)
 {
-    if (n < 0 || lda < (n > 0 ? n : 1) || !info) return TMVX_ERR_ARG;
-    *info = 0;
-    if (n == 0) return TMVX_OK;
-    if (!A || !piv) return TMVX_ERR_ARG;
+    if (n < 0 || lda < (n > 0 ? n : 1) || (!info && !na) || (na && !na->d_info))
+        return TMVX_ERR_ARG;
+    if (info) *info = 0;
+    if (n > 0 && (!A || !piv)) return TMVX_ERR_ARG;
+    if (n == 0) {
+        if (!na) return TMVX_OK;
+        hipLaunchKernelGGL(getrf_finish_kernel, dim3(1), dim3(64), 0, s, nullptr, nullptr, n,
+                           na->d_info);
+        return hip_status(hipGetLastError());
+    }
     constexpr int NB = PANEL_NB, BB = PANEL_BB, NTH = PANEL_THREADS;
     typedef ColSlot<T, BB> CS;
     const int Gmax = (int)((n + NTH - 1) / NTH);
@@ -713,7 +728,12 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
     const size_t szts = tslu ? sizeof(int) * (size_t)tslu_ws_ints(n) : 0;
     const size_t total = szslots + 256 + szts;
     char* ws = nullptr;
-    if (hipMallocAsync((void**)&ws, total, s) != hipSuccess) return TMVX_ERR_ALLOC;
+    if (na) {
+        if (total > na->bytes || tslu || lowprec != TMVX_LOWPREC_F32) return TMVX_ERR_UNSUPPORTED;
+        ws = static_cast<char*>(na->ws);
+    } else if (hipMallocAsync((void**)&ws, total, s) != hipSuccess) {
+        return TMVX_ERR_ALLOC;
+    }
     PanelWork wk;
     wk.cslots = reinterpret_cast<unsigned long long*>(ws);
     wk.bslots = wk.cslots + ncslot;
@@ -825,7 +845,10 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
         }
         if (rc != TMVX_OK) break;
         if (side_pending) {
-            if (hipStreamWaitEvent(s, ev_side, 0) != hipSuccess) { rc = TMVX_ERR_HIP; break; }
+            if (side != s && hipStreamWaitEvent(s, ev_side, 0) != hipSuccess) {
+                rc = TMVX_ERR_HIP;
+                break;
+            }
             side_pending = false;
         }
         // Deferred interchanges on the columns outside the outer panel.  With
@@ -846,15 +869,20 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
         rc = trailing(nr, JE2 - JE, JE - J0, A + JE + J0 * lda, A + J0 + JE * lda,
                       A + JE + JE * lda, bmax_main, s);
         if (rc != TMVX_OK || JE2 >= n) continue;
-        // the rest of the trailing matrix on the side stream
+        // the rest of the trailing matrix on the side stream (without
+        // allocation: the same kernels in the same order on s itself)
         if (!side) {
-            if ((rc = side_acquire(&ss)) != TMVX_OK) break;
-            side = ss->stream;
-            ev_main = ss->ev_main;
-            ev_side = ss->ev_side;
+            if (na) {
+                side = s;
+            } else {
+                if ((rc = side_acquire(&ss)) != TMVX_OK) break;
+                side = ss->stream;
+                ev_main = ss->ev_main;
+                ev_side = ss->ev_side;
+            }
         }
-        if (hipEventRecord(ev_main, s) != hipSuccess ||
-            hipStreamWaitEvent(side, ev_main, 0) != hipSuccess) {
+        if (side != s && (hipEventRecord(ev_main, s) != hipSuccess ||
+                          hipStreamWaitEvent(side, ev_main, 0) != hipSuccess)) {
             rc = TMVX_ERR_HIP;
             break;
         }
@@ -867,8 +895,14 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
         rc = trailing(nr, n - JE2, JE - J0, A + JE + J0 * lda, A + J0 + JE2 * lda,
                       A + JE + JE2 * lda, bmax_side, side);
         if (rc != TMVX_OK) break;
-        if (hipEventRecord(ev_side, side) != hipSuccess) { rc = TMVX_ERR_HIP; break; }
+        if (side != s && hipEventRecord(ev_side, side) != hipSuccess) { rc = TMVX_ERR_HIP; break; }
         side_pending = true;
+    }
+    if (na) {                              // asynchronous: info stays on the device
+        if (rc != TMVX_OK) return rc;
+        hipLaunchKernelGGL(getrf_finish_kernel, dim3(1), dim3(64), 0, s, wk.err, wk.first_zero, n,
+                           na->d_info);
+        return hip_status(hipGetLastError());
     }
     // whatever happened above, the caller's stream waits for the side stream
     if (side) {
@@ -881,7 +915,7 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
     if (hipMemcpyAsync(&herr, wk.err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipMemcpyAsync(&fz, wk.first_zero, sizeof(i64), hipMemcpyDeviceToHost, s) != hipSuccess)
         rc = TMVX_ERR_HIP;
-    hipFreeAsync(ws, s);
+    (void)hipFreeAsync(ws, s);
     const bool sync_ok = hipStreamSynchronize(s) == hipSuccess;
     side_release(ss);                      // its work is done: s waited for it
     if (!sync_ok) return TMVX_ERR_HIP;
@@ -976,8 +1010,38 @@ int piv_to_perm(i64 n, const i64* piv, i64* perm, hipStream_t s)
     return hip_status(hipGetLastError());
 }
 
-template int getrf_impl<double>(i64, double*, i64, i64*, i64*, i64*, hipStream_t, int, int);
-template int getrf_impl<float>(i64, float*, i64, i64*, i64*, i64*, hipStream_t, int, int);
+template int getrf_impl<double>(i64, double*, i64, i64*, i64*, i64*, hipStream_t, int, int,
+                                const GetrfNoAlloc*);
+template int getrf_impl<float>(i64, float*, i64, i64*, i64*, i64*, hipStream_t, int, int,
+                               const GetrfNoAlloc*);
+
+// Scratch of the no-allocation factorisation: the panel exchange slots and
+// flags in a static device array (each GPU holds its own copy for the
+// life of the process).  Host code never uses the array's name as an
+// address: the device address comes from hipGetSymbolAddress, once per
+// device, with that device current.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 64 * 1024;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+}
+
+static int scratch_address(void** out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return TMVX_ERR_HIP;
+    if (dev < 0 || dev >= 64) return TMVX_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return TMVX_ERR_HIP;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TMVX_OK;
+}
 
 } // namespace tmvx
 
@@ -997,6 +1061,20 @@ int tmvx_getrf_f64(int64_t n, double* A, int64_t lda, int64_t* piv, int64_t* inf
 { return tmvx::getrf_impl<double>(n, A, lda, piv, nullptr, info, (hipStream_t)stream); }
 int tmvx_getrf_f32(int64_t n, float* A, int64_t lda, int64_t* piv, int64_t* info, void* stream)
 { return tmvx::getrf_impl<float>(n, A, lda, piv, nullptr, info, (hipStream_t)stream); }
+
+int tmvx_getrf_f64_noalloc(int64_t n, double* A, int64_t lda, int64_t* piv, int64_t* d_info,
+                           void* stream)
+{
+    if (n > 65536) return TMVX_ERR_UNSUPPORTED;              // the scratch array's limit
+    if (n < 0 || lda < (n > 0 ? n : 1) || !d_info || (n > 0 && (!A || !piv)))
+        return TMVX_ERR_ARG;
+    void* ws = nullptr;
+    const int rc = tmvx::scratch_address(&ws);
+    if (rc != TMVX_OK) return rc;
+    const tmvx::GetrfNoAlloc na{ws, tmvx::NOALLOC_SCRATCH_BYTES, d_info};
+    return tmvx::getrf_impl<double>(n, A, lda, piv, nullptr, nullptr, (hipStream_t)stream,
+                                    TMVX_LOWPREC_F32, 0, &na);
+}
 
 int tmvx_getrf_f32_tournament(int64_t n, float* A, int64_t lda, int64_t* piv, int64_t* info,
                               void* stream)
