@@ -483,6 +483,34 @@ def unpack_lu(LU):
     return L, U
 
 
+def trsm(A, B, upper, trans=False, unit=False, dtype=np.float64, lib: _Lib | None = None):
+    """op(A)^-1 B for triangular A (TMV TriLDivEq / NonBlasTriLDivEq,
+    src/TMV_TriDiv_M.cpp:190-266): the recursive split restated in
+    orc_trsm_*; trans solves with A^T (the transpose is formed here, so the
+    lower and upper roles swap).  Raises SingularError on a zero diagonal
+    met by a non-zero entry."""
+    lib = lib or oracle_lib()
+    dtype = np.dtype(dtype)
+    suf, _ = _SUF[dtype]
+    A = np.asarray(A, dtype)
+    if trans:
+        A, upper = A.T, not upper
+    A = _fortran(A, dtype)
+    n = A.shape[0]
+    B = np.asarray(B, dtype)
+    vec = B.ndim == 1
+    X = _fcopy(B.reshape(n, -1), dtype)
+    f = lib.fn("trsm_" + suf)
+    f.argtypes = [ctypes.c_int, ctypes.c_int, _i64, _i64, _vp, _i64, _vp, _i64]
+    rc = f(int(bool(upper)), int(bool(unit)), n, X.shape[1], _ptr(A), max(1, n), _ptr(X),
+           max(1, n))
+    if rc == -2:
+        raise SingularError("singular triangle")
+    if rc != 0:
+        raise RuntimeError(f"trsm failed: {rc}")
+    return X[:, 0] if vec else X
+
+
 def apply_perm_rows(A, P):
     """Apply the interchange list in order (TMV permuteRows)."""
     A = np.array(A, copy=True)

@@ -7,9 +7,11 @@ EPS64 = np.finfo(np.float64).eps
 EPS32 = np.finfo(np.float32).eps
 
 
-def gemm_bound_ok(C, Cref, A, B, k, eps, factor=2.0, C0=None, beta=0.0, alpha=1.0):
+def gemm_bound_ok(C, Cref, A, B, k, eps, factor=1.0, C0=None, beta=0.0, alpha=1.0):
     """Elementwise |C - Cref| <= factor * k * eps * (|alpha| |A||B| + |beta||C0|)
-    (+ a tiny absolute floor for exact zeros)."""
+    (+ a tiny absolute floor for exact zeros).  factor = 1 is the stated
+    k eps |A||B| bound against an exact (long-double) product; two
+    working-precision results each within it differ by at most factor 2."""
     mag = abs(alpha) * (np.abs(A) @ np.abs(B))
     if C0 is not None and beta != 0:
         mag = mag + abs(beta) * np.abs(C0)

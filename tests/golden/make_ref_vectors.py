@@ -42,6 +42,25 @@ ZGEMM_VARIANTS = [("n_n_real", 0, 0, 1.0, False), ("c_n_add", 1, 0, -0.5, True),
                   ("n_c_cplx", 0, 1, 0.3 + 0.7j, False), ("c_c_add", 1, 1, 2.0, True)]
 
 
+HADAMARD_SIZES = (64, 256, 1024)
+
+
+def hadamard_rows(n: int) -> np.ndarray:
+    """Sylvester-Hadamard matrix of order n (a power of 2) with its rows in
+    the order argsort(u), u uniform from the documented generator."""
+    H = np.ones((1, 1))
+    while H.shape[0] < n:
+        H = np.block([[H, H], [H, -H]])
+    return H[np.argsort(rng.uniform(SEED + 600 + n, n), kind="stable")]
+
+
+def cfg2_matrix() -> np.ndarray:
+    """The n = 8192 uniform [-1,1] matrix of the plain FP64 configuration at
+    scale: tmvx_fill_uniform_f64(seed + 50) over n*n values read column-major."""
+    n = 8192
+    return rng.uniform(SEED + 50, n * n, -1.0, 1.0).reshape(n, n).T.copy()
+
+
 def main() -> None:
     ref = oracle.ref_lib()
     if ref is None:
@@ -113,6 +132,26 @@ def main() -> None:
     b = rng.uniform(SEED + 2, n, -1.0, 1.0)
     v["cfg1_x"] = oracle.ref_gesv(As, b)
     v["cfg1_logdet"] = np.array(oracle.ref_logdet(As))
+    # Exact ties: Sylvester-Hadamard matrices with seeded row orders.  Every
+    # elimination step is an exact tie among +-|pivot| candidates and all
+    # arithmetic is exact, so TMV's first-index rule alone fixes the pivots
+    # and the factors are exact (digest of the column-major bytes).
+    for n in HADAMARD_SIZES:
+        H = hadamard_rows(n)
+        LU, P = oracle.getrf(H, lib=ref)
+        v[f"hadamard_{n}_P"] = P
+        v[f"hadamard_{n}_LU_sha256"] = np.array(digest(LU))
+    # The plain FP64 configuration at scale: LU of the n = 8192 uniform
+    # [-1,1] matrix (no shift) -- TMV's whole pivot sequence, and the
+    # smallest relative gap between the two largest candidates over the 8192
+    # steps (from the restatement, whose pivots equal TMV's).
+    n = 8192
+    A = cfg2_matrix()
+    _, P = oracle.getrf(A, lib=ref)
+    _, Po, gaps = oracle.getrf(A, gaps=True)
+    assert np.array_equal(P, Po)
+    v["cfg2_lu_P"] = P
+    v["cfg2_lu_min_gap"] = np.array(gaps.min())
     np.savez_compressed(OUT, **v)
     print(f"wrote {OUT}: {len(v)} arrays, {os.path.getsize(OUT)} bytes")
 

@@ -271,5 +271,5 @@ def test_gemm_batched(dev, dtype):
         C0 = Cs[i].to(dtype).double().numpy()
         ref = -0.5 * (np.asarray(A, np.longdouble) @ np.asarray(B, np.longdouble)) + 2.0 * C0
         ok, worst = gemm_bound_ok(dC[i].double().cpu().numpy(), ref, A, B, int(k[i]), eps,
-                                  C0=C0, beta=2.0, alpha=-0.5)
+                                  factor=1.0, C0=C0, beta=2.0, alpha=-0.5)
         assert ok, (i, worst)

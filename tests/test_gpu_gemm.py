@@ -30,6 +30,11 @@ def test_dgemm_parity(orc, dev, m, n, k, ar, br):
     print(f"dgemm m={m} n={n} k={k} layout={ar}{br}", flush=True)
     A, B, Ad, Bd = _operands(m, n, k, ar, br, torch.float64)
     C = to_host(tmv_amd.gemm(Ad, Bd))
+    # the stated bound k eps |A||B| against the exact (long-double) product
+    Cl = A.astype(np.longdouble) @ B.astype(np.longdouble)
+    ok, worst = gemm_bound_ok(C, Cl, A, B, k, EPS64, factor=1.0)
+    assert ok, worst
+    # and the oracle (TMV's summation order), each within that bound
     Co = orc.gemm(A, B, a_rowmajor=ar, b_rowmajor=br)
     ok, worst = gemm_bound_ok(C, Co, A, B, k, EPS64, factor=2.0)
     assert ok, worst
@@ -45,7 +50,7 @@ def test_dgemm_add_alpha(orc, dev, m, n, k):
         tmv_amd.gemm(Ad, Bd, Cd, alpha=alpha, beta=beta)
         C = to_host(Cd)
         Cl = beta * C0.astype(np.longdouble) + alpha * (A.astype(np.longdouble) @ B.astype(np.longdouble))
-        ok, worst = gemm_bound_ok(C, Cl, A, B, k, EPS64, 2.0, C0, beta, alpha)
+        ok, worst = gemm_bound_ok(C, Cl, A, B, k, EPS64, 1.0, C0, beta, alpha)
         assert ok, (alpha, beta, worst)
 
 
@@ -86,9 +91,12 @@ def test_sgemm_parity(orc, dev, m, n, k, ar, br):
     A, B, Ad, Bd = _operands(m, n, k, ar, br, torch.float32)
     A32, B32 = A.astype(np.float32), B.astype(np.float32)
     C = to_host(tmv_amd.gemm(Ad, Bd))
+    A64, B64 = A32.astype(np.float64), B32.astype(np.float64)
+    Cl = A64.astype(np.longdouble) @ B64.astype(np.longdouble)
+    ok, worst = gemm_bound_ok(C, Cl, A64, B64, k, EPS32, factor=1.0)
+    assert ok, worst
     Co = orc.gemm(A32, B32, a_rowmajor=ar, b_rowmajor=br, dtype=np.float32)
-    ok, worst = gemm_bound_ok(C, Co, A32.astype(np.float64), B32.astype(np.float64), k,
-                              EPS32, factor=2.0)
+    ok, worst = gemm_bound_ok(C, Co, A64, B64, k, EPS32, factor=2.0)
     assert ok, worst
 
 

@@ -495,3 +495,72 @@ def test_live_reference_agrees(orc):
     LU, P = orc.getrf(A)
     LU2, P2 = orc.getrf(A, lib=ref)
     assert np.array_equal(P, P2)
+
+
+# ------------------------------------------------- exact ties, triangles ----
+
+@pytest.mark.parametrize("n", [64, 256, 1024])
+def test_hadamard_ties_match_tmv(orc, ref_vectors, n):
+    """Row-shuffled Sylvester-Hadamard matrices: every elimination step is an
+    exact tie (gap 0) and the arithmetic is exact, so TMV's first-index rule
+    (src/TMV_Vector.cpp:473-504) alone fixes the pivots.  The restatement
+    gives TMV's pivots and bit-identical factors; a last-index rule does
+    not."""
+    import make_ref_vectors as M
+    H = M.hadamard_rows(n)
+    LU, P, gaps = orc.getrf(H, gaps=True)
+    assert np.array_equal(P, ref_vectors[f"hadamard_{n}_P"])
+    assert hashlib.sha256(np.asfortranarray(LU).tobytes(order="F")).hexdigest() == \
+        str(ref_vectors[f"hadamard_{n}_LU_sha256"])
+    assert (gaps[:-1] == 0).all()                      # ties at every step
+    # a planted last-index rule chooses differently on the first tie
+    col = np.abs(H[:, 0])
+    assert int(np.nonzero(col == col.max())[0][-1]) != int(P[0])
+
+
+def test_cfg2_fixture_is_tie_free(ref_vectors):
+    """The stored n = 8192 pivot sequence is tie-free well beyond 64 n eps,
+    so GPU pivots must equal it at every step."""
+    n = 8192
+    assert ref_vectors["cfg2_lu_P"].shape == (n,)
+    assert float(ref_vectors["cfg2_lu_min_gap"]) > 1e3 * 64 * n * EPS64
+
+
+def _tri(n, upper, seed, zero_diag=None):
+    A = rng.uniform_matrix(seed, n, n) / max(n, 1)
+    d = 1.0 + rng.uniform(seed + 1, n, 0.0, 1.0)
+    A[np.diag_indices(n)] = d * np.where(rng.uniform(seed + 2, n) < 0.5, -1.0, 1.0)
+    A = np.triu(A) if upper else np.tril(A)
+    if zero_diag is not None:
+        A[zero_diag, zero_diag] = 0.0
+    return A
+
+
+@pytest.mark.parametrize("upper", [0, 1])
+@pytest.mark.parametrize("trans", [0, 1])
+@pytest.mark.parametrize("unit", [0, 1])
+def test_trsm_vs_lapack(orc, upper, trans, unit):
+    """The restated recursive TRSM (src/TMV_TriDiv_M.cpp:190-266) against
+    LAPACK's trsm (scipy) for every triangle/transpose/unit combination; a
+    solve with the wrong triangle fails the same check."""
+    n, nrhs = 150, 7
+    A = _tri(n, upper, SEED + 700 + 4 * upper + 2 * trans + unit)
+    B = rng.uniform_matrix(SEED + 710, n, nrhs)
+    X = orc.trsm(A, B, upper=upper, trans=trans, unit=unit)
+    Xl = sla.solve_triangular(A, B, lower=not upper, trans=trans, unit_diagonal=bool(unit))
+    assert np.abs(X - Xl).max() <= 64 * n * EPS64 * np.abs(Xl).max()
+    Xw = orc.trsm(A.T, B, upper=not upper, trans=trans, unit=unit)     # wrong triangle
+    assert np.abs(Xw - Xl).max() > 1e-3 * np.abs(Xl).max()
+
+
+def test_trsm_zero_diagonal(orc):
+    """A zero diagonal met by a non-zero entry raises (TMV's
+    SingularUpperTriMatrix); met only by zero entries it does not
+    (src/TMV_TriDiv_V.cpp:171-217 skips zero entries)."""
+    n = 40
+    A = _tri(n, 1, SEED + 720, zero_diag=10)
+    with pytest.raises(orc.SingularError):
+        orc.trsm(A, rng.uniform(SEED + 721, n, -1, 1), upper=1)
+    b = np.zeros(n)
+    b[:5] = 1.0                                    # rows 0..4 only: row 10 stays 0
+    orc.trsm(A, b, upper=1)
