@@ -2,7 +2,7 @@
 """Headline benchmark: LU-solve TFLOP/s, FP32-MFMA + iterative refinement vs
 plain FP64, at a normwise backward error <= 1e-12; DGEMM % of FP64 peak.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--only SECTION]
 
 One step = one mixed-precision solve x = b/A of the mixed-precision
 configuration: A = U diag(s) V^T, n = 16384, U and V orthogonal factors of
@@ -13,12 +13,18 @@ FP64 residual and update on the GPU until backward error <= 1e-12.
 solve (the HPL-MxP convention), inputs resident in HBM.
 
 Also measured in the same run (outside the K timed steps):
-  - the plain FP64 GPU LU solve (getrf + getrs) on the same A and b;
-  - the same solver with FP16 / BF16 (and FP32) trailing updates on a
-    kappa = 1e2 matrix of the same construction;
-  - DGEMM M = N = K = 8192 in the CCC, CRC and RCC operand layouts; the
-    CCC kernel is the `roofline` entry (FP64 MFMA, 78.6 TFLOP/s peak),
-    timed with HIP events on the stream it runs on;
+  - `roofline_step`: the dominant kernel of the step, the FP32 panel
+    factorisation (panel_kernel<float>), timed with HIP events on the
+    stream it runs on during one more solve; algorithmic bytes = the panel
+    read once and written once (8 m jb bytes for an m x jb FP32 panel);
+  - the mixed solver over kappa in {1e2, 1e4, 1e6}: FP32 factorisation
+    with 1 and 64 right-hand sides, FP16 and BF16 trailing updates with
+    GMRES-IR, each against the plain FP64 GPU LU solve on the same A and b;
+  - the plain-FP64 configurations: DGEMM M = N = K = 8192 in the CCC, CRC
+    and RCC operand layouts (CCC is the `roofline` entry, FP64 MFMA,
+    78.6 TFLOP/s peak, HIP events on the stream it runs on) and the FP64
+    LU n = 8192 with 16 right-hand sides (uniform[-1,1], b ~ N(0,1));
+    the CPU-sized case: DGEMM 512^3 and the LU solve n = 512 (A + n I);
   - the batched workload: 20,000 systems n ~ U{8..128} (+ n on the
     diagonal), nrhs ~ U{1..4}, FP64 and FP32, factor + solve in one
     hipGraph-replayed persistent launch (autotuned plan) vs per-matrix
@@ -27,9 +33,14 @@ Also measured in the same run (outside the K timed steps):
     the adjoint (grad_b = A^-T g, grad_A = -grad_b x^T) with stored and
     with recomputed factors;
   - the CPU baseline: the real TMV 0.76 library (oracle/_ref, built from
-    the reference sources) on a bounded sample, x = b/A at n = 4096 with
-    OpenMP on the host's cores (kind "reference"); the oracle restatement
-    (kind "port") where the TMV build is absent.
+    the reference sources) on the host's cores (OpenMP), best of 3 with a
+    cache flush between runs: x = b/A at n = 4096 (the bounded sample of
+    the headline workload, `cpu_baseline`), and the 512 case (GEMM and LU
+    solve); --cpu-cfg2 adds the 8192 case (three GEMM layouts, LU + 16
+    right-hand sides; minutes of CPU time).  The oracle restatement (kind
+    "port") stands in where the TMV build is absent.
+--only mixed|fp64|batched|cpu runs one section (profiling); the JSON line
+then carries that section's figures only.
 Prints ONE JSON line.
 """
 from __future__ import annotations
@@ -45,11 +56,16 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
 SEED = 0x7A15EED
 # HBM bytes per launch of the 8192^3 CCC DGEMM (gemm_kernel f64 128x128x16),
 # from rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the factors a
 # 1 GiB device copy reads on the same counters; profiles/dgemm_8192_ccc_traffic.md.
 DGEMM_8192_CCC_TRAFFIC = 13.71e9
+# HBM bytes per launch of the FP32 panel kernel (64-column panels), averaged
+# over the 256 panels of an n = 16384 LU (same counters and factors);
+# profiles/panel_f32_n16384_traffic.md.
+PANEL_F32_16384_TRAFFIC = {64: 6.25e6}
 
 
 def conditioned_gpu(n, kappa, seed, torch, tmv_amd):
@@ -71,35 +87,60 @@ def conditioned_gpu(n, kappa, seed, torch, tmv_amd):
 
 
 def backward_error_gpu(A, x, b, torch):
-    r = b - A @ x
+    """max over columns of ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf)."""
+    X = x if x.dim() > 1 else x.view(-1, 1)
+    B = b if b.dim() > 1 else b.view(-1, 1)
+    r = B - A @ X
     nA = A.abs().sum(dim=1).max()
-    return float(r.abs().max() / (nA * x.abs().max() + b.abs().max()))
+    be = r.abs().max(dim=0).values / (nA * X.abs().max(dim=0).values + B.abs().max(dim=0).values)
+    return float(be.max())
+
+
+def _flush_cache(np, buf=[None]):
+    """Touch 256 MiB so that the next CPU run starts from cold caches (as
+    TMV's speed harness does between runs)."""
+    if buf[0] is None:
+        buf[0] = np.ones(32 << 20)
+    buf[0] += 1.0
+
+
+def _best_of(np, fn, reps, cap_s):
+    best = None
+    t_start = time.perf_counter()
+    for _ in range(reps):
+        _flush_cache(np)
+        t0 = time.perf_counter()
+        fn()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+        if time.perf_counter() - t_start > cap_s:
+            break
+    return best
+
+
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _cpu_lib():
+    import oracle
+    lib = oracle.ref_lib()
+    if lib is None:
+        oracle.build()
+    return oracle, lib, ("reference" if lib is not None else "port")
 
 
 def cpu_baseline(seconds_cap=30.0):
-    """TMV's own CPU path on a bounded sample: x = b/A, n = 4096, best of 3."""
+    """TMV's own CPU path on a bounded sample of the headline workload:
+    x = b/A, n = 4096, best of 3."""
     import numpy as np
-    import oracle
     from tmv_amd import rng
+    oracle, lib, kind = _cpu_lib()
     n = 4096
     A = rng.uniform_matrix(SEED + 4000, n, n)
     b = rng.uniform(SEED + 4001, n, -1.0, 1.0)
-    lib = oracle.ref_lib()
-    kind = "reference" if lib is not None else "port"
-    if lib is None:
-        oracle.build()
-    best = None
-    t_start = time.perf_counter()
-    for _ in range(3):
-        t0 = time.perf_counter()
-        if lib is not None:
-            oracle.ref_gesv(A, b)
-        else:
-            oracle.gesv(A, b)
-        dt = time.perf_counter() - t0
-        best = dt if best is None else min(best, dt)
-        if time.perf_counter() - t_start > seconds_cap:
-            break
+    solve = (lambda: oracle.ref_gesv(A, b)) if lib is not None else (lambda: oracle.gesv(A, b))
+    best = _best_of(np, solve, 3, seconds_cap)
     flops = 2.0 / 3.0 * n ** 3 + 2.0 * n * n
     threads = (lib or oracle.oracle_lib()).fn("num_threads")()
     return {"value": round(flops / best / 1e12, 5), "unit": "TFLOP/s", "cores": int(threads),
@@ -107,6 +148,51 @@ def cpu_baseline(seconds_cap=30.0):
             "sample": (f"x = b/A (divideUsing LU: LU_Decompose + LU_LDivEq, OpenMP MultMM), "
                        f"n={n}, uniform[-1,1] seeded, best of 3, {best * 1e3:.0f} ms"),
             "ms": round(best * 1e3, 1)}
+
+
+def cpu_configs(cfg2=False, reps=3):
+    """TMV on the plain-FP64 configurations: DGEMM 512^3 and the LU solve
+    n = 512 (A + n I, 1 right-hand side); with cfg2 also DGEMM 8192^3 in
+    the CCC, CRC and RCC layouts and the LU n = 8192 with 16 right-hand
+    sides.  Best of `reps` (each run capped in total time), cache flushed
+    between runs."""
+    import numpy as np
+    from tmv_amd import rng
+    oracle, lib, kind = _cpu_lib()
+    threads = int((lib or oracle.oracle_lib()).fn("num_threads")())
+    out = {"kind": kind, "cores": threads, "reps": reps}
+    gesv = oracle.ref_gesv if lib is not None else oracle.gesv
+
+    n = 512
+    A = np.asfortranarray(rng.uniform_matrix(SEED + 100, n, n))
+    B = np.asfortranarray(rng.uniform_matrix(SEED + 101, n, n))
+    t = _best_of(np, lambda: oracle.gemm(A, B, lib=lib), reps, 20.0)
+    out["dgemm_512"] = {"us": round(t * 1e6, 1), "gflops": round(2 * n ** 3 / t / 1e9, 2)}
+    A = A + n * np.eye(n)
+    b = rng.uniform(SEED + 102, n, -1.0, 1.0)
+    t = _best_of(np, lambda: gesv(A, b), reps, 20.0)
+    out["lu_solve_512"] = {"us": round(t * 1e6, 1),
+                           "gflops": round((2 / 3 * n ** 3 + 2 * n * n) / t / 1e9, 2)}
+    if cfg2:
+        n = 8192
+        Ac = np.asfortranarray(rng.uniform_matrix(SEED + 200, n, n))
+        Bc = np.asfortranarray(rng.uniform_matrix(SEED + 201, n, n))
+        Ar = np.ascontiguousarray(Ac)
+        Br = np.ascontiguousarray(Bc)
+        for name, a_, b_, arm, brm in (("CCC", Ac, Bc, False, False),
+                                       ("CRC", Ar, Bc, True, False),
+                                       ("RCC", Ac, Br, False, True)):
+            _progress(f"TMV DGEMM 8192 {name}")
+            t = _best_of(np, lambda: oracle.gemm(a_, b_, a_rowmajor=arm, b_rowmajor=brm, lib=lib),
+                         reps, 60.0)
+            out[f"dgemm_8192_{name}"] = {"s": round(t, 3), "gflops": round(2 * n ** 3 / t / 1e9, 2)}
+        del Ar, Br, Bc
+        Bn = np.asfortranarray(np.random.default_rng(SEED + 202).standard_normal((n, 16)))
+        _progress("TMV LU 8192, 16 RHS")
+        t = _best_of(np, lambda: gesv(Ac, Bn), reps, 60.0)
+        out["lu_8192_nrhs16"] = {"s": round(t, 3),
+                                 "gflops": round((2 / 3 * n ** 3 + 2 * n * n * 16) / t / 1e9, 2)}
+    return out
 
 
 def batched_workload(torch, tmv_amd, dtype, count=20000, reps=5, per_matrix_sample=200):
@@ -291,44 +377,82 @@ def adjoint_workload(torch, tmv_amd, dtype, count=512, n=256, reps=3):
             "adjoint_recompute_ms": round(min(rec) * 1e3, 2)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=16384)
-    ap.add_argument("--kappa", type=float, default=1e4)
-    ap.add_argument("--nrhs", type=int, default=1)
-    ap.add_argument("--fp64-reps", type=int, default=2)
-    ap.add_argument("--gemm-n", type=int, default=8192)
-    ap.add_argument("--gemm-reps", type=int, default=5)
-    ap.add_argument("--zgemm-n", type=int, default=8192, help="complex GEMM size (0: skip)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--no-lowp", dest="lowp", action="store_false")
-    ap.add_argument("--no-grid", dest="grid", action="store_false",
-                    help="skip the kappa x nrhs grid of mixed solves")
-    ap.add_argument("--only-batched", action="store_true")
-    a = ap.parse_args()
-    if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
-              "design is described in DESIGN.md and is not measured", file=sys.stderr)
-        sys.exit(2)
+def _timed(torch, fn, reps=1):
+    """Best wall time of fn() over reps runs, device synchronised around each."""
+    best = None
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return best
 
-    import torch
-    import tmv_amd
-    tmv_amd.lib()                      # the HIP library; no fallback
+
+def _event_ms(torch, fn, reps):
+    """Average HIP-event time of fn() on the current stream over reps runs."""
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def panel_roofline(torch, tmv_amd, A, b, n):
+    """The step's dominant kernel, the FP32 panel factorisation: one more
+    mixed solve with HIP events around every panel launch (on the stream
+    the panel runs on).  Algorithmic bytes: each m x jb FP32 panel read once
+    and written once, 8 m jb bytes."""
+    tmv_amd.panel_timer_start(4096)
+    try:
+        tmv_amd.gesv_mixed(A, b, tol=1e-12, max_iter=50)
+        torch.cuda.synchronize()
+    finally:
+        launches, total_ms = tmv_amd.panel_timer_stop()
+    nb = next((w for w in (128, 64) if -(-n // w) == launches), 64)   # panel width
+    bytes_ = sum(8.0 * (n - j0) * min(nb, n - j0) for j0 in range(0, n, nb))
+    per_launch = bytes_ / launches
+    avg_ms = total_ms / launches
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 6),
+            "traffic": PANEL_F32_16384_TRAFFIC.get(nb) if n == 16384 else None,
+            "kernel": f"panel_kernel<float,{nb},8,512> (FP32 panel of the n={n} mixed solve)",
+            "launches": launches, "avg_us": round(avg_ms * 1e3, 2),
+            "total_ms": round(total_ms, 3), "bytes_per_launch": round(per_launch),
+            "share_of_step": None}
+
+
+def fp64_lu_solve(torch, tmv_amd, A, b, reps, parts=False):
+    """Plain FP64 GPU LU solve (getrf + getrs) on a copy of A: best wall time
+    (with parts: also the getrf and getrs times of that run)."""
+    best, be, split = None, None, None
+    for _ in range(reps):
+        LU = A.clone()
+        x = b.clone()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        piv, _ = tmv_amd.getrf(LU)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        tmv_amd.getrs(LU, piv, x)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        if best is None or t3 - t1 < best:
+            best, split = t3 - t1, (t2 - t1, t3 - t2)
+        be = backward_error_gpu(A, x, b, torch)
+        del LU, x
+    return (best, be, split) if parts else (best, be)
+
+
+def mixed_section(torch, tmv_amd, a):
     n, nrhs = a.n, a.nrhs
-    if a.only_batched:                 # the batched workloads alone (profiling)
-        print(json.dumps({"batched_20000": {
-            "f64": batched_workload(torch, tmv_amd, torch.float64),
-            "f32": batched_workload(torch, tmv_amd, torch.float32)},
-            "adjoint_512x256": {
-            "f64": adjoint_workload(torch, tmv_amd, torch.float64),
-            "f32": adjoint_workload(torch, tmv_amd, torch.float32)}}), flush=True)
-        return
-
-    # ---- inputs (untimed) -------------------------------------------------
     t_setup = time.perf_counter()
     A = conditioned_gpu(n, a.kappa, SEED + 10, torch, tmv_amd)
     b = torch.empty(n * nrhs, dtype=torch.float64, device="cuda")
@@ -350,90 +474,76 @@ def main():
     torch.cuda.synchronize()
     t_mixed = (time.perf_counter() - t0) / a.steps
     flops = 2.0 / 3.0 * n ** 3 + 2.0 * n * n * nrhs
-    value = flops / t_mixed / 1e12
     be_mixed = backward_error_gpu(A, X, b, torch)
+    del X
 
-    # ---- FP16 / BF16 factorisations (kappa = 1e2, where classical
-    #      refinement from a 16-bit trailing update converges) --------------
-    lowp = {}
-    if a.lowp:
-        A2 = conditioned_gpu(n, 1e2, SEED + 14, torch, tmv_amd)
-        for name, lp in (("f16", tmv_amd.LOWPREC_F16), ("bf16", tmv_amd.LOWPREC_BF16),
-                         ("f32", tmv_amd.LOWPREC_F32)):
-            tmv_amd.gesv_mixed(A2, b, tol=1e-12, max_iter=60, lowprec=lp, check=False)
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            Xl, inf = tmv_amd.gesv_mixed(A2, b, tol=1e-12, max_iter=60, lowprec=lp, check=False)
-            torch.cuda.synchronize()
-            dt = time.perf_counter() - t1
-            lowp[name] = {"ms": round(dt * 1e3, 3), "tflops": round(flops / dt / 1e12, 3),
-                          "iterations": inf["iterations"], "converged": inf["converged"],
-                          "backward_error": backward_error_gpu(A2, Xl, b, torch),
-                          "factor_ms": round(inf["factor_ms"], 3),
-                          "refine_ms": round(inf["refine_ms"], 3)}
-        del Xl
-    else:
-        A2 = None
+    roof = panel_roofline(torch, tmv_amd, A, b, n)
+    roof["share_of_step"] = round(roof["total_ms"] / (t_mixed * 1e3), 3)
 
-    # ---- the configuration grid: kappa in {1e2, 1e4, 1e6} x nrhs in {1, 64},
-    #      FP32 factorisation + FP64 refinement, one untimed-warm run each ---
-    grid = None
+    t_fp64, be64 = fp64_lu_solve(torch, tmv_amd, A, b, a.fp64_reps)
+    out = {"value": flops / t_mixed / 1e12, "t_mixed": t_mixed, "setup_s": setup_s,
+           "mixed": {"backward_error": be_mixed, "iterations": infos[-1]["iterations"],
+                     "converged": infos[-1]["converged"],
+                     "factor_ms": round(infos[-1]["factor_ms"], 3),
+                     "refine_ms": round(infos[-1]["refine_ms"], 3)},
+           "fp64_lu_solve": {"ms": round(t_fp64 * 1e3, 3),
+                             "tflops": round(flops / t_fp64 / 1e12, 3),
+                             "backward_error": be64},
+           "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
+           "roofline_step": roof}
+
+    _progress(f"mixed: {t_mixed * 1e3:.1f} ms/step, panel {roof['avg_us']} us/launch, "
+              f"FP64 LU solve {t_fp64 * 1e3:.1f} ms")
+    # ---- kappa in {1e2, 1e4, 1e6}: FP32 (1 and 64 RHS), FP16, BF16 -------
     if a.grid:
         grid = {}
         b64 = torch.empty(n * 64, dtype=torch.float64, device="cuda")
         tmv_amd.fill_normal(b64, SEED + 16)
         b64 = b64.view(64, n).t()                 # column-major n x 64
         b1 = b if nrhs == 1 else b[:, 0].contiguous()
-        mats = [(1e2, A2 if A2 is not None else conditioned_gpu(n, 1e2, SEED + 14, torch, tmv_amd)),
-                (a.kappa, A), (1e6, conditioned_gpu(n, 1e6, SEED + 18, torch, tmv_amd))]
-        for kappa, M in mats:
-            for nr, rhs in ((1, b1), (64, b64)):
-                tmv_amd.gesv_mixed(M, rhs, tol=1e-12, max_iter=50, check=False)
+        legs = [("f32", tmv_amd.LOWPREC_F32, 1), ("f32", tmv_amd.LOWPREC_F32, 64),
+                ("f16", tmv_amd.LOWPREC_F16, 1), ("bf16", tmv_amd.LOWPREC_BF16, 1)]
+        for kappa, seed in ((1e2, SEED + 14), (a.kappa, None), (1e6, SEED + 18)):
+            M = A if seed is None else conditioned_gpu(n, kappa, seed, torch, tmv_amd)
+            for name, lp, nr in legs:
+                rhs = b1 if nr == 1 else b64
+                it = 50 if lp == tmv_amd.LOWPREC_F32 else a.lowp_iter
+                tmv_amd.gesv_mixed(M, rhs, tol=1e-12, max_iter=it, lowprec=lp, check=False)
                 torch.cuda.synchronize()
                 t1 = time.perf_counter()
-                Xg, inf = tmv_amd.gesv_mixed(M, rhs, tol=1e-12, max_iter=50, check=False)
+                Xg, inf = tmv_amd.gesv_mixed(M, rhs, tol=1e-12, max_iter=it, lowprec=lp,
+                                             check=False)
                 torch.cuda.synchronize()
                 dt = time.perf_counter() - t1
                 fl = 2.0 / 3.0 * n ** 3 + 2.0 * n * n * nr
-                grid[f"kappa{kappa:.0e}_nrhs{nr}"] = {
+                grid[f"kappa{kappa:.0e}_{name}_nrhs{nr}"] = {
                     "ms": round(dt * 1e3, 3), "tflops": round(fl / dt / 1e12, 3),
-                    "iterations": inf["iterations"], "converged": inf["converged"],
+                    "refine": "classic" if lp == tmv_amd.LOWPREC_F32 else "gmres",
+                    "iterations": inf["iterations"], "inner_iterations": inf["inner_iterations"],
+                    "converged": inf["converged"], "status": inf["status"],
+                    "factor_ms": round(inf["factor_ms"], 3),
+                    "refine_ms": round(inf["refine_ms"], 3),
                     "backward_error": backward_error_gpu(M, Xg, rhs, torch)}
                 del Xg
-        # the FP64 LU solve with 64 right-hand sides, for the ratio at nrhs = 64
-        LU = A.clone()
-        x64 = b64.clone()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        piv, _ = tmv_amd.getrf(LU)
-        tmv_amd.getrs(LU, piv, x64)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t1
-        grid["fp64_lu_solve_nrhs64"] = {"ms": round(dt * 1e3, 3),
-                                        "backward_error": backward_error_gpu(A, x64, b64, torch)}
-        del LU, x64, b64, mats
-    A2 = None
-    torch.cuda.empty_cache()
-
-    # ---- plain FP64 GPU LU solve on the same A, b ---------------------------
-    t64 = []
-    be64 = None
-    for _ in range(a.fp64_reps):
-        LU = A.clone()
-        x64 = b.clone()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        piv, finfo = tmv_amd.getrf(LU)
-        tmv_amd.getrs(LU, piv, x64)
-        torch.cuda.synchronize()
-        t64.append(time.perf_counter() - t1)
-        be64 = backward_error_gpu(A, x64, b, torch)
-        del LU
-    t_fp64 = min(t64)
+                _progress(f"kappa {kappa:.0e} {name} nrhs {nr}: {dt * 1e3:.1f} ms, "
+                          f"converged {inf['converged']}")
+            for nr, rhs in ((1, b1), (64, b64)):
+                t64, be = fp64_lu_solve(torch, tmv_amd, M, rhs, 1)
+                grid[f"kappa{kappa:.0e}_fp64_nrhs{nr}"] = {"ms": round(t64 * 1e3, 3),
+                                                          "backward_error": be}
+            del M
+            torch.cuda.empty_cache()
+        out["mixed_grid"] = grid
+        del b64
     del A
     torch.cuda.empty_cache()
+    return out
 
-    # ---- DGEMM 8192^3 (CCC, CRC, RCC) with HIP events -----------------------
+
+def fp64_section(torch, tmv_amd, a):
+    """DGEMM 8192^3 (CCC, CRC, RCC) and the LU n = 8192 with 16 right-hand
+    sides; DGEMM 512^3 and the LU solve n = 512; complex GEMM."""
+    out = {}
     gn = a.gemm_n
     GA = torch.empty(gn * gn, dtype=torch.float64, device="cuda")
     GB = torch.empty(gn * gn, dtype=torch.float64, device="cuda")
@@ -444,26 +554,60 @@ def main():
     for lay in ("CC", "CR", "RC"):
         Av = GA.view(gn, gn).t() if lay[0] == "C" else GA.view(gn, gn)
         Bv = GB.view(gn, gn).t() if lay[1] == "C" else GB.view(gn, gn)
-        tmv_amd.gemm(Av, Bv, GC)
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(a.gemm_reps):
-            tmv_amd.gemm(Av, Bv, GC)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / a.gemm_reps
+        ms = _event_ms(torch, lambda: tmv_amd.gemm(Av, Bv, GC), a.gemm_reps)
         tf = 2.0 * gn ** 3 / ms / 1e9
         name = {"CC": "CCC", "CR": "CRC", "RC": "RCC"}[lay]
         gemm[name] = {"ms": round(ms, 3), "tflops": round(tf, 2),
                       "frac_peak": round(tf / PEAK_FP64_TFLOPS, 4)}
-    del GA, GB, GC
+    out["dgemm_8192"] = gemm
+    _progress(f"dgemm 8192: {gemm}")
+    del GB, GC
+    torch.cuda.empty_cache()
+
+    # LU n = 8192, 16 right-hand sides: A uniform[-1,1] (no shift), b ~ N(0,1)
+    n = 8192
+    A = GA.view(n, n).t() if gn == n else None
+    if A is None:
+        A = torch.empty(n * n, dtype=torch.float64, device="cuda")
+        tmv_amd.fill_uniform(A, SEED + 20)
+        A = A.view(n, n).t()
+    B16 = torch.empty(n * 16, dtype=torch.float64, device="cuda")
+    tmv_amd.fill_normal(B16, SEED + 24)
+    B16 = B16.view(16, n).t()
+    t, be, (tf, ts) = fp64_lu_solve(torch, tmv_amd, A, B16, 2, parts=True)
+    out["lu_8192_nrhs16"] = {"ms": round(t * 1e3, 3),
+                             "tflops": round((2 / 3 * n ** 3 + 2 * n * n * 16) / t / 1e12, 3),
+                             "getrf_ms": round(tf * 1e3, 3), "getrs_ms": round(ts * 1e3, 3),
+                             "backward_error": be}
+    del A, GA, B16
+    torch.cuda.empty_cache()
+
+    # the CPU-sized case: DGEMM 512^3, LU solve n = 512 (A + n I, 1 RHS)
+    n = 512
+    A = torch.empty(n * n, dtype=torch.float64, device="cuda")
+    B = torch.empty(n * n, dtype=torch.float64, device="cuda")
+    tmv_amd.fill_uniform(A, SEED + 100)
+    tmv_amd.fill_uniform(B, SEED + 101)
+    Av, Bv = A.view(n, n).t(), B.view(n, n).t()
+    C = tmv_amd.empty_colmajor(n, n, torch.float64)
+    g_us = _event_ms(torch, lambda: tmv_amd.gemm(Av, Bv, C), 200) * 1e3
+    g_wall = _timed(torch, lambda: tmv_amd.gemm(Av, Bv, C), 20) * 1e6
+    M = Av.clone()
+    M.diagonal().add_(float(n))
+    bb = torch.empty(n, dtype=torch.float64, device="cuda")
+    tmv_amd.fill_uniform(bb, SEED + 102)
+    t, be, (tf, ts) = fp64_lu_solve(torch, tmv_amd, M, bb, 20, parts=True)
+    out["cfg512"] = {"dgemm_us": round(g_us, 2), "dgemm_wall_us": round(g_wall, 2),
+                     "dgemm_tflops": round(2 * n ** 3 / g_us / 1e6, 3),
+                     "lu_solve_us": round(t * 1e6, 1), "getrf_us": round(tf * 1e6, 1),
+                     "getrs_us": round(ts * 1e6, 1),
+                     "lu_solve_gflops": round((2 / 3 * n ** 3 + 2 * n * n) / t / 1e9, 2),
+                     "lu_backward_error": be}
+    del A, B, C, M
     torch.cuda.empty_cache()
 
     # ---- complex GEMM (TMV's four-real-product split), complex128 CCC ------
     zn = a.zgemm_n
-    zgemm = None
     if zn > 0:
         ZA = torch.empty(2 * zn * zn, dtype=torch.float64, device="cuda")
         ZB = torch.empty(2 * zn * zn, dtype=torch.float64, device="cuda")
@@ -472,22 +616,74 @@ def main():
         ZAv = torch.view_as_complex(ZA.view(zn, zn, 2)).t()
         ZBv = torch.view_as_complex(ZB.view(zn, zn, 2)).t()
         ZC = tmv_amd.empty_colmajor(zn, zn, torch.complex128)
-        tmv_amd.gemm_complex(ZAv, ZBv, ZC)
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(a.gemm_reps):
-            tmv_amd.gemm_complex(ZAv, ZBv, ZC)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / a.gemm_reps
+        ms = _event_ms(torch, lambda: tmv_amd.gemm_complex(ZAv, ZBv, ZC), a.gemm_reps)
         tf = 8.0 * zn ** 3 / ms / 1e9
-        zgemm = {"n": zn, "ms": round(ms, 3), "tflops": round(tf, 2),
-                 "frac_peak": round(tf / PEAK_FP64_TFLOPS, 4),
-                 "flops": "8 n^3 real (4 real products)"}
+        out["zgemm"] = {"n": zn, "ms": round(ms, 3), "tflops": round(tf, 2),
+                        "frac_peak": round(tf / PEAK_FP64_TFLOPS, 4),
+                        "flops": "8 n^3 real (4 real products)"}
         del ZA, ZB, ZC, ZAv, ZBv
         torch.cuda.empty_cache()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--kappa", type=float, default=1e4)
+    ap.add_argument("--nrhs", type=int, default=1)
+    ap.add_argument("--fp64-reps", type=int, default=2)
+    ap.add_argument("--lowp-iter", type=int, default=4,
+                    help="outer refinement steps for the FP16/BF16 legs (GMRES-IR)")
+    ap.add_argument("--gemm-n", type=int, default=8192)
+    ap.add_argument("--gemm-reps", type=int, default=5)
+    ap.add_argument("--zgemm-n", type=int, default=8192, help="complex GEMM size (0: skip)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-cfg2", action="store_true",
+                    help="also time TMV on the 8192 case (minutes of CPU)")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-grid", dest="grid", action="store_false",
+                    help="skip the kappa grid of mixed solves")
+    ap.add_argument("--only", choices=["mixed", "fp64", "batched", "cpu"], default=None)
+    ap.add_argument("--only-batched", action="store_true", help="same as --only batched")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): every figure is "
+              "one GPU's; the multi-GPU layout is a design, not measured", file=sys.stderr)
+        sys.exit(2)
+    if a.only_batched:
+        a.only = "batched"
+
+    if a.only == "cpu":                 # TMV on the host alone (no GPU)
+        print(json.dumps({"cpu_baseline": cpu_baseline(),
+                          "cpu_configs": cpu_configs(a.cpu_cfg2, a.cpu_reps)}), flush=True)
+        return
+
+    import torch
+    import tmv_amd
+    tmv_amd.lib()                      # the HIP library; no fallback
+    if a.only == "batched":            # the batched workloads alone (profiling)
+        print(json.dumps({"batched_20000": {
+            "f64": batched_workload(torch, tmv_amd, torch.float64),
+            "f32": batched_workload(torch, tmv_amd, torch.float32)},
+            "adjoint_512x256": {
+            "f64": adjoint_workload(torch, tmv_amd, torch.float64),
+            "f32": adjoint_workload(torch, tmv_amd, torch.float32)}}), flush=True)
+        return
+    if a.only == "fp64":
+        print(json.dumps(fp64_section(torch, tmv_amd, a)), flush=True)
+        return
+
+    mx = mixed_section(torch, tmv_amd, a)
+    if a.only == "mixed":
+        mx.pop("t_mixed")
+        print(json.dumps(mx), flush=True)
+        return
+    f64 = fp64_section(torch, tmv_amd, a)
+    _progress("fp64 section done")
 
     batched = adjoint = None
     if not a.no_batched:
@@ -496,43 +692,45 @@ def main():
         adjoint = {"f64": adjoint_workload(torch, tmv_amd, torch.float64),
                    "f32": adjoint_workload(torch, tmv_amd, torch.float32)}
 
-    cpu = None
+    _progress("GPU sections done")
+    cpu = cpu_cfg = None
     if not a.no_cpu_baseline:
         try:
             cpu = cpu_baseline()
         except Exception as ex:          # reported, never silently replaced
             cpu = {"value": None, "unit": "TFLOP/s", "cores": None, "kind": "reference",
                    "sample": f"failed: {ex}"}
+        try:
+            cpu_cfg = cpu_configs(a.cpu_cfg2, a.cpu_reps)
+        except Exception as ex:
+            cpu_cfg = {"failed": str(ex)}
 
-    ccc = gemm["CCC"]
+    ccc = f64["dgemm_8192"]["CCC"]
+    gn = a.gemm_n
     out = {
         "metric": "LU-solve TFLOP/s: FP32-MFMA+IR vs FP64 at <=1e-12 backward err; DGEMM % peak",
-        "value": round(value, 3),
+        "value": round(mx["value"], 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(t_mixed * 1e3, 3),
+        "ms_per_step": round(mx["t_mixed"] * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 factorisation + f64 refinement",
         "data": "synthetic: A = U diag(s) V^T (QR of seeded Gaussians), b ~ N(0,1)",
         "config": {"workload": "mixed-precision LU solve + iterative refinement",
-                   "n": n, "nrhs": nrhs, "kappa": a.kappa, "target_backward_error": 1e-12,
+                   "n": a.n, "nrhs": a.nrhs, "kappa": a.kappa, "target_backward_error": 1e-12,
                    "parallelism": "single GPU"},
-        "mixed": {"backward_error": be_mixed,
-                  "iterations": infos[-1]["iterations"],
-                  "factor_ms": round(infos[-1]["factor_ms"], 3),
-                  "refine_ms": round(infos[-1]["refine_ms"], 3)},
-        "fp64_lu_solve": {"ms": round(t_fp64 * 1e3, 3),
-                          "tflops": round(flops / t_fp64 / 1e12, 3),
-                          "backward_error": be64},
-        "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
-        "mixed_kappa1e2": lowp or None,
-        "mixed_grid": grid,
-        "dgemm_8192": gemm,
-        "zgemm": zgemm,
+        "mixed": mx["mixed"],
+        "fp64_lu_solve": mx["fp64_lu_solve"],
+        "speedup_vs_fp64": mx["speedup_vs_fp64"],
+        "mixed_grid": mx.get("mixed_grid"),
+        "dgemm_8192": f64["dgemm_8192"],
+        "lu_8192_nrhs16": f64["lu_8192_nrhs16"],
+        "cfg512": f64["cfg512"],
+        "zgemm": f64.get("zgemm"),
         "batched_20000": batched,
         "adjoint_512x256": adjoint,
         "roofline": {"bound": "mfma", "achieved": ccc["tflops"], "peak": PEAK_FP64_TFLOPS,
@@ -540,8 +738,10 @@ def main():
                      "traffic": DGEMM_8192_CCC_TRAFFIC if gn == 8192 else None,
                      "kernel": "gemm_kernel f64 128x128x16 (DGEMM 8192^3, CCC)",
                      "flops_per_launch": 2 * gn ** 3},
+        "roofline_step": mx["roofline_step"],
         "cpu_baseline": cpu,
-        "setup_s": round(setup_s, 1),
+        "cpu_configs": cpu_cfg,
+        "setup_s": round(mx["setup_s"], 1),
     }
     print(json.dumps(out), flush=True)
 

@@ -85,6 +85,20 @@ int tmvx_getrf_f32(int64_t n, float* A, int64_t lda, int64_t* piv,
                    int64_t* info, void* stream);
 
 /*
+ * The same for an m x n column-major A (lda >= m): min(m, n) pivot steps,
+ * piv has min(m, n) entries; a wide A's columns past m take the row
+ * interchanges and U = L^-1 A there, a tall A's rows past n end as L.
+ * *info as above (first zero pivot among the min(m, n)).  Synchronises.
+ * Replaces: tmv::LU_Decompose on a non-square MatrixView (src/
+ * TMV_LUDecompose.cpp:217-245,514-538: RecursiveLUDecompose with
+ * R = min(M, N)) and its dgetrf seam (src/TMV_LUDecompose.cpp:418-442).
+ */
+int tmvx_getrf_mn_f64(int64_t m, int64_t n, double* A, int64_t lda, int64_t* piv,
+                      int64_t* info, void* stream);
+int tmvx_getrf_mn_f32(int64_t m, int64_t n, float* A, int64_t lda, int64_t* piv,
+                      int64_t* info, void* stream);
+
+/*
  * tmvx_getrf_f64 that allocates nothing: no counterpart in TMV; for callers
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  The same factorisation
@@ -241,6 +255,23 @@ int tmvx_logdet_f64(int64_t n, const double* LU, int64_t lda, const int64_t* piv
                     double* logdet, double* sign, void* stream);
 int tmvx_logdet_f32(int64_t n, const float* LU, int64_t lda, const int64_t* piv,
                     double* logdet, double* sign, void* stream);
+
+/*
+ * min_i |U_ii| and max_i |U_ii| of the packed factors (for complex
+ * entries |re| + |im|, TMV's Abs2), written to host memory (the call
+ * synchronises the stream).  n = 0 gives min = +inf, max = 0.
+ * Replaces: the diagonal scan of LUDiv::isSingular (src/TMV_LUD.cpp:
+ * 195-200: minAbs2Element <= eps * maxAbs2Element).  _z / _c take
+ * interleaved complex storage as tmvx_getrf_z / _c write it.
+ */
+int tmvx_lu_diag_range_f64(int64_t n, const double* LU, int64_t lda, double* dmin,
+                           double* dmax, void* stream);
+int tmvx_lu_diag_range_f32(int64_t n, const float* LU, int64_t lda, double* dmin,
+                           double* dmax, void* stream);
+int tmvx_lu_diag_range_z(int64_t n, const double* LU, int64_t lda, double* dmin,
+                         double* dmax, void* stream);
+int tmvx_lu_diag_range_c(int64_t n, const float* LU, int64_t lda, double* dmin,
+                         double* dmax, void* stream);
 
 /*
  * B := op(A)^-1 B with A n x n triangular (upper != 0: upper), unit != 0

@@ -37,6 +37,18 @@ int tmvx_gemm_cfg_f32(int cfg, int layout_a, int layout_b, int64_t m, int64_t n,
 int tmvx_getrf_f32_tournament(int64_t n, float* A, int64_t lda, int64_t* piv,
                               int64_t* info, void* stream);
 
+/*
+ * Panel timer (diagnostics; no counterpart in TMV).  While started, every
+ * panel-kernel launch of the allocating factorisations (tmvx_getrf_*, and
+ * through them the solvers) is bracketed by a pair of HIP events recorded
+ * on the stream the panel runs on, for the first `cap` launches.  _stop
+ * waits for the recorded events and returns the launch count and the summed
+ * event-to-event time in milliseconds.  One timer per process: start/stop
+ * pairs do not nest; TMVX_ERR_ARG for a second start or a stop without one.
+ */
+int tmvx_panel_timer_start(int cap);
+int tmvx_panel_timer_stop(int64_t* launches, double* total_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -195,9 +195,13 @@ def getrf(A, dtype=np.float64, gaps=False, lib: _Lib | None = None):
     if lib.prefix == "orc_":
         f.argtypes = [_i64, _i64, _vp, _i64, _vp, _vp]
         rc = f(m, n, _ptr(LU), max(1, m), _ptr(P), _ptr(g) if gaps else None)
+    elif m != n:
+        f = lib.fn("getrf_mn_" + suf)
+        f.argtypes = [_i64, _i64, _vp, _i64, _vp]
+        P = np.zeros(max(m, n), np.int64)     # TMV writes at most max(m, n) entries
+        rc = f(m, n, _ptr(LU), max(1, m), _ptr(P))
+        P = P[:min(m, n)].copy()
     else:
-        if m != n:
-            raise ValueError("reference driver takes square matrices")
         f.argtypes = [_i64, _vp, _i64, _vp]
         rc = f(n, _ptr(LU), max(1, n), _ptr(P))
     if rc != 0:

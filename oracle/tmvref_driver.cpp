@@ -109,6 +109,16 @@ int getrf(ptrdiff_t n, T* A, ptrdiff_t lda, ptrdiff_t* P)
     return 0;
 }
 
+// The same on an m x n view (R = min(m, n) pivots).
+template <class T>
+int getrf_mn(ptrdiff_t m, ptrdiff_t n, T* A, ptrdiff_t lda, ptrdiff_t* P)
+{
+    try {
+        tmv::LU_Decompose(mview(A, m, n, lda, 0), P);
+    } catch (...) { return -1; }
+    return 0;
+}
+
 // B := (P L U)^-1 B through tmv::LU_LDivEq (src/TMV_LUDiv.cpp:185-226);
 // trans != 0 solves with the transpose through LU_RDivEq on B^T
 // (src/TMV_LUDiv.cpp:369-410): x^T (P L U) = b^T.
@@ -230,6 +240,10 @@ int tmvref_getrf_f64(ptrdiff_t n, double* A, ptrdiff_t lda, ptrdiff_t* P)
 { return getrf<double>(n, A, lda, P); }
 int tmvref_getrf_f32(ptrdiff_t n, float* A, ptrdiff_t lda, ptrdiff_t* P)
 { return getrf<float>(n, A, lda, P); }
+int tmvref_getrf_mn_f64(ptrdiff_t m, ptrdiff_t n, double* A, ptrdiff_t lda, ptrdiff_t* P)
+{ return getrf_mn<double>(m, n, A, lda, P); }
+int tmvref_getrf_mn_f32(ptrdiff_t m, ptrdiff_t n, float* A, ptrdiff_t lda, ptrdiff_t* P)
+{ return getrf_mn<float>(m, n, A, lda, P); }
 
 int tmvref_getrs_f64(int trans, ptrdiff_t n, ptrdiff_t nrhs, const double* LU,
                      ptrdiff_t lda, const ptrdiff_t* P, double* B, ptrdiff_t ldb)

@@ -36,6 +36,23 @@ def lu_residual(A, LU, P):
     return float(np.abs(R).max() / max(np.abs(A).max(), 1e-300))
 
 
+def lu_residual_mn(A, LU, P):
+    """||P A - L U||_max / ||A||_max for an m x n factorisation (L m x r
+    unit-lower, U r x n upper, r = min(m, n)), in long double."""
+    A = np.asarray(A, np.longdouble)
+    LU = np.asarray(LU, np.longdouble)
+    m, n = A.shape
+    r = min(m, n)
+    L = np.tril(LU[:, :r], -1) + np.eye(m, r, dtype=np.longdouble)
+    U = np.triu(LU[:r, :])
+    PA = A.copy()
+    for i, p in enumerate(np.asarray(P)):
+        if p != i:
+            PA[[i, p]] = PA[[p, i]]
+    R = PA - L @ U
+    return float(np.abs(R).max() / max(np.abs(A).max(), 1e-300))
+
+
 def backward_error(A, x, b):
     """||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf), in long double."""
     A = np.asarray(A, np.longdouble)

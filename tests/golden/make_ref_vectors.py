@@ -43,6 +43,7 @@ ZGEMM_VARIANTS = [("n_n_real", 0, 0, 1.0, False), ("c_n_add", 1, 0, -0.5, True),
 
 
 HADAMARD_SIZES = (64, 256, 1024)
+RECT_SHAPES = ((5, 3), (3, 5), (100, 37), (37, 100), (300, 130), (130, 300), (1, 7), (7, 1))
 
 
 def hadamard_rows(n: int) -> np.ndarray:
@@ -110,6 +111,12 @@ def main() -> None:
         # m.inverse() and m.logDet(&sign) with LU division.
         v[f"inv_{n}"] = oracle.ref_inverse(M)
         v[f"logdet_{n}"] = np.array(oracle.ref_logdet(M))
+    # LU_Decompose on non-square views (R = min(m, n) pivots): tall and wide.
+    for (m, n) in RECT_SHAPES:
+        M = rng.uniform_matrix(SEED + 700 + m + 3 * n, m, n)
+        LU, P = oracle.getrf(M, lib=ref)
+        v[f"lurect_{m}x{n}_P"] = P
+        v[f"lurect_{m}x{n}_LU"] = LU
     # TMV's own fixtures (tests/TMV_TestMatrixDecomp.cpp): pivots + digest.
     for t in (0, 3, 4, 5, 6):
         LU, P = oracle.getrf(F.decomp_matrix(t), lib=ref)

@@ -151,3 +151,30 @@ def test_empty_matrix(dev):
     piv = torch.empty(0, dtype=torch.int64, device=dev)
     assert tmv_amd.logdet(LU, piv) == (0.0, 1.0)
     assert tmv_amd.getri(LU, piv).shape == (0, 0)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.complex128,
+                                   torch.complex64])
+@pytest.mark.parametrize("n", [1, 7, 300, 2049])
+def test_lu_diag_range(dev, dtype, n):
+    """min / max Abs2(U_ii) (|re| + |im| for complex, TMV's isSingular
+    measure, src/TMV_LUD.cpp:195-200) reduced on the device equal the host's
+    scan of the same diagonal, exactly (a min / max picks an element)."""
+    g = np.random.default_rng(n)
+    M = g.uniform(-1, 1, (n, n))
+    if dtype.is_complex:
+        M = M + 1j * g.uniform(-1, 1, (n, n))
+    M[n // 2, n // 2] = 0.0                       # one zero on the diagonal
+    d = tmv_amd.colmajor(torch.from_numpy(M).to(dtype).cuda())
+    mn, mx = tmv_amd.lu_diag_range(d)
+    dg = np.diag(d.cpu().numpy())
+    if dtype.is_complex:                          # summed in FP64, as the kernel does
+        a = np.abs(dg.real).astype(np.float64) + np.abs(dg.imag).astype(np.float64)
+    else:
+        a = np.abs(dg).astype(np.float64)
+    assert mn == a.min() and mx == a.max()
+
+
+def test_lu_diag_range_empty(dev):
+    mn, mx = tmv_amd.lu_diag_range(tmv_amd.colmajor(torch.empty(0, 0, dtype=torch.float64).cuda()))
+    assert mn == float("inf") and mx == 0.0

@@ -2,13 +2,15 @@
 (TMV LU_Decompose / LU_LDivEq restated).  Bar: identical pivots on
 tie-free inputs, ||P A - L U|| <= 8 n eps ||A||, solve backward error
 <= 8 n eps."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
 
 import fixtures as F
 import tmv_amd
-from checks import EPS64, EPS32, lu_residual, backward_error
+from checks import EPS64, EPS32, lu_residual, lu_residual_mn, backward_error
 from devutil import to_dev, to_host
 from tmv_amd import rng
 
@@ -207,3 +209,97 @@ def test_getrs_interchange_lists(dev, n, kind):
     X = to_dev(B)
     tmv_amd.getrs(to_dev(np.eye(n)), torch.from_numpy(piv.astype(np.int64)).cuda(), X)
     assert np.array_equal(to_host(X), _swapped(piv, B))
+
+
+RECT = [(5, 3), (3, 5), (100, 37), (37, 100), (300, 130), (130, 300), (1, 7), (7, 1)]
+
+
+@pytest.mark.parametrize("m,n", RECT)
+def test_getrf_mn_matches_tmv(orc, dev, ref_vectors, m, n):
+    """LU_Decompose of a non-square matrix (tmvx_getrf_mn_f64): the pivots
+    are TMV's (fixture written by TMV itself) up to the first near tie,
+    P A = L U to 8 max(m, n) eps, and the first min(m, n) columns of the
+    factors are TMV's to the same bound.  Past column m of a wide matrix
+    the factors are L^-1 P A, as TMV's dgetrf seam gives them; TMV's native
+    path leaves those columns off (its P A - L U is O(1) there)."""
+    print(f"getrf_mn {m}x{n}", flush=True)
+    A = rng.uniform_matrix(SEED + 700 + m + 3 * n, m, n)
+    _, _, gaps = orc.getrf(A, gaps=True)
+    Ad = to_dev(A)
+    piv, info = tmv_amd.getrf_mn(Ad)
+    P, LU = piv.cpu().numpy(), to_host(Ad)
+    r = min(m, n)
+    assert info == 0 and P.shape == (r,)
+    ok = gaps > 64 * max(m, n) * EPS64
+    k0 = r if ok.all() else int(np.argmin(ok))
+    assert np.array_equal(P[:k0], ref_vectors[f"lurect_{m}x{n}_P"][:k0])
+    bound = 8 * max(m, n) * EPS64
+    assert lu_residual_mn(A, LU, P) <= bound
+    if k0 == r:
+        LUt = ref_vectors[f"lurect_{m}x{n}_LU"][:, :r]
+        assert np.abs(LU[:, :r] - LUt).max() <= bound * max(1.0, np.abs(LUt).max())
+
+
+@pytest.mark.parametrize("m,n", [(2000, 700), (700, 2000), (1030, 1029), (64, 4000)])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_getrf_mn_large(orc, dev, m, n, dtype):
+    """Larger tall and wide matrices (several outer panels, the side stream,
+    columns past m taking interchanges and U only) against the oracle."""
+    print(f"getrf_mn {m}x{n} {dtype}", flush=True)
+    A = rng.uniform_matrix(SEED + 710 + m + n, m, n)
+    eps = EPS64 if dtype == torch.float64 else EPS32
+    Ao = A if dtype == torch.float64 else A.astype(np.float32)
+    _, Po, gaps = orc.getrf(Ao, dtype=np.float64 if dtype == torch.float64 else np.float32,
+                             gaps=True)
+    Ad = to_dev(Ao, dtype)
+    piv, info = tmv_amd.getrf_mn(Ad)
+    P, LU = piv.cpu().numpy(), to_host(Ad)
+    assert info == 0
+    ok = gaps > 64 * max(m, n) * eps
+    k0 = min(m, n) if ok.all() else int(np.argmin(ok))
+    if dtype == torch.float64:            # FP32 gaps fall below 64 n eps32 early
+        assert k0 > min(m, n) // 4
+    assert np.array_equal(P[:k0], Po[:k0])
+    assert lu_residual_mn(Ao, LU, P) <= 8 * max(m, n) * eps
+
+
+def test_getrf_mn_square_is_getrf(dev):
+    """m == n through the rectangular entry is the square factorisation,
+    bit for bit (the same kernels in the same order)."""
+    n = 700
+    A = rng.uniform_matrix(SEED + 720, n, n)
+    A1, A2 = to_dev(A), to_dev(A)
+    p1, _ = tmv_amd.getrf(A1)
+    p2, _ = tmv_amd.getrf_mn(A2)
+    assert torch.equal(p1, p2) and torch.equal(A1, A2)
+
+
+def test_getrf_mn_empty_and_args(dev):
+    for (m, n) in ((0, 5), (5, 0), (0, 0)):
+        piv, info = tmv_amd.getrf_mn(tmv_amd.colmajor(torch.zeros(m, n, dtype=torch.float64).cuda()))
+        assert info == 0 and piv.numel() == 0
+    info = ctypes.c_int64(0)                                  # lda < m
+    assert tmv_amd.lib().tmvx_getrf_mn_f64(5, 3, None, 4, None, ctypes.byref(info),
+                                           None) == tmv_amd.ERR_ARG
+
+
+def test_getrf_n40000_f32(dev):
+    """n = 40000 (past the 32768 limit of earlier versions): FP32 LU and a
+    solve, backward error <= 8 n eps32 in FP64 arithmetic on the device."""
+    n = 40000
+    print(f"getrf f32 n={n}", flush=True)
+    A64 = torch.empty(n * n, dtype=torch.float64, device="cuda")
+    tmv_amd.fill_uniform(A64, SEED + 730)
+    A = A64.view(n, n).t().to(torch.float32)                 # column-major FP32
+    del A64
+    LU = A.clone()
+    piv, info = tmv_amd.getrf(LU)
+    assert info == 0
+    b = torch.ones(n, dtype=torch.float32, device="cuda")
+    x = b.clone()
+    tmv_amd.getrs(LU, piv, x)
+    del LU
+    Ad = A.double()
+    r = b.double() - Ad @ x.double()
+    be = float(r.abs().max() / (Ad.abs().sum(dim=1).max() * x.double().abs().max() + 1.0))
+    assert be <= 8 * n * EPS32, be

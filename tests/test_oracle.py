@@ -564,3 +564,45 @@ def test_trsm_zero_diagonal(orc):
     b = np.zeros(n)
     b[:5] = 1.0                                    # rows 0..4 only: row 10 stays 0
     orc.trsm(A, b, upper=1)
+
+
+# ------------------------------------------------ non-square LU views ---
+
+RECT = [(5, 3), (3, 5), (100, 37), (37, 100), (300, 130), (130, 300), (1, 7), (7, 1)]
+
+
+@pytest.mark.parametrize("m,n", RECT)
+def test_getrf_rect_matches_tmv(orc, ref_vectors, m, n):
+    """The restatement on non-square matrices (R = min(m, n) pivots,
+    src/TMV_LUDecompose.cpp:217-245): TMV's pivots exactly and TMV's factors
+    to rounding.  P A = L U holds on the first min(m, n) columns; for a wide
+    matrix TMV's native path leaves the columns past m off L^-1 P A (its
+    dgetrf seam, src/TMV_LUDecompose.cpp:418-442, does not), and the
+    restatement follows TMV there too."""
+    from checks import EPS64, lu_residual_mn
+    A = rng.uniform_matrix(SEED + 700 + m + 3 * n, m, n)
+    LU, P = orc.getrf(A)
+    assert np.array_equal(P, ref_vectors[f"lurect_{m}x{n}_P"])
+    LUt = ref_vectors[f"lurect_{m}x{n}_LU"]
+    bound = 8 * max(m, n) * EPS64
+    r = min(m, n)
+    assert lu_residual_mn(A[:, :r], LU[:, :r], P) <= bound
+    assert lu_residual_mn(A[:, :r], LUt[:, :r], P) <= bound
+    assert np.abs(LU - LUt).max() <= bound * max(1.0, np.abs(LUt).max())
+    if m >= n:
+        assert lu_residual_mn(A, LU, P) <= bound
+
+
+def test_lu_residual_mn_catches_errors(orc):
+    """The rectangular residual check fails on a planted mistake: a swapped
+    interchange, or a wrong multiplier."""
+    from checks import EPS64, lu_residual_mn
+    A = rng.uniform_matrix(SEED + 705, 60, 25)
+    LU, P = orc.getrf(A)
+    assert lu_residual_mn(A, LU, P) <= 8 * 60 * EPS64
+    Pb = P.copy()
+    Pb[3] = (Pb[3] + 7) % 60 if Pb[3] + 7 >= 3 else Pb[3] + 7
+    assert lu_residual_mn(A, LU, Pb) > 1e-3
+    LUb = LU.copy()
+    LUb[40, 10] += 1e-6
+    assert lu_residual_mn(A, LUb, P) > 8 * 60 * EPS64

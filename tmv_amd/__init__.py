@@ -84,6 +84,14 @@ _SIGS = {
     "tmvx_gemm_batched_f64": (_int, [_i64, _vp, _vp, _vp, _d, _vp, _vp, _d, _vp, _vp]),
     "tmvx_gemm_batched_f32": (_int, [_i64, _vp, _vp, _vp, _f, _vp, _vp, _f, _vp, _vp]),
     "tmvx_getrf_f32_tournament": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
+    "tmvx_lu_diag_range_f64": (_int, [_i64, _vp, _i64, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
+    "tmvx_lu_diag_range_f32": (_int, [_i64, _vp, _i64, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
+    "tmvx_lu_diag_range_z": (_int, [_i64, _vp, _i64, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
+    "tmvx_lu_diag_range_c": (_int, [_i64, _vp, _i64, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
+    "tmvx_getrf_mn_f64": (_int, [_i64, _i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
+    "tmvx_getrf_mn_f32": (_int, [_i64, _i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
+    "tmvx_panel_timer_start": (_int, [_int]),
+    "tmvx_panel_timer_stop": (_int, [ctypes.POINTER(_i64), ctypes.POINTER(_d)]),
     "tmvx_gemm_f32_lowp": (_int, [_int, _i64, _i64, _i64, _f, _vp, _i64, _vp, _i64, _f, _vp, _i64, _int, _vp]),
     "tmvx_getrf_batched_f64": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "tmvx_getrf_batched_f32": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
@@ -176,6 +184,8 @@ def _layout(t, name: str):
         raise ValueError(f"{name} must be 2-D")
     m, n = t.shape
     s0, s1 = t.stride()
+    if m == 0 or n == 0:                       # no elements: any layout
+        return COLMAJOR, max(1, m)
     if s0 == 1 and (n <= 1 or s1 >= max(1, m)):
         return COLMAJOR, max(s1, m, 1)
     if s1 == 1 and (m <= 1 or s0 >= max(1, n)):
@@ -321,6 +331,25 @@ def getrf(A, stream=None):
     return piv, int(info.value)
 
 
+def getrf_mn(A, stream=None):
+    """In-place PA = LU of an m x n column-major tensor (TMV LU_Decompose on
+    a non-square view).  Returns (piv, info): min(m, n) 0-based interchanges
+    and the 1-based first zero pivot (0 if none)."""
+    torch = _torch()
+    if A.dim() != 2:
+        raise ValueError(f"A must be a 2-D tensor (got shape {tuple(A.shape)})")
+    m, n = A.shape
+    suf = _suffix(A.dtype)
+    lda = _colmajor_ld(A, "A")
+    _on_gpu(A)
+    piv = torch.empty(min(m, n), dtype=torch.int64, device=A.device)
+    info = _i64(0)
+    fn = getattr(lib(), "tmvx_getrf_mn_" + suf)
+    _check(fn(m, n, A.data_ptr(), lda, piv.data_ptr(), ctypes.byref(info), _stream(stream)),
+           "getrf_mn")
+    return piv, int(info.value)
+
+
 def getrf_complex(A, stream=None):
     """In-place PA = LU of a square column-major complex128 / complex64
     tensor (TMV LU_Decompose for complex T).  Returns (piv, info)."""
@@ -429,6 +458,20 @@ def getrf_tournament(A, stream=None):
     return piv, int(info.value)
 
 
+def panel_timer_start(cap=4096):
+    """Record HIP events around the next `cap` panel-kernel launches of the
+    factorisations (on the stream each panel runs on)."""
+    _check(lib().tmvx_panel_timer_start(int(cap)), "panel_timer_start")
+
+
+def panel_timer_stop():
+    """Stop the panel timer: (launches, summed launch time in ms)."""
+    cnt = _i64(0)
+    ms = _d(0.0)
+    _check(lib().tmvx_panel_timer_stop(ctypes.byref(cnt), ctypes.byref(ms)), "panel_timer_stop")
+    return int(cnt.value), float(ms.value)
+
+
 def getrs(LU, piv, B, trans=False, check_singular=True, stream=None):
     """In-place solve with the factors: B := A^-1 B (trans: A^-T B).
     B is column-major n x nrhs (a 1-D tensor is treated as n x 1)."""
@@ -458,6 +501,24 @@ def getri(LU, piv, stream=None):
     _check(fn(n, LU.data_ptr(), lda, piv.data_ptr(), X.data_ptr(), max(n, 1), _stream(stream)),
            "getri")
     return X
+
+
+def lu_diag_range(LU, stream=None):
+    """(min_i |U_ii|, max_i |U_ii|) of packed factors, |z| = |re| + |im| for
+    complex (the numbers TMV's LUDiv::isSingular compares)."""
+    torch = _torch()
+    n = _square(LU, "LU")
+    lda = _colmajor_ld(LU, "LU")
+    _on_gpu(LU)
+    suf = {torch.float64: "f64", torch.float32: "f32", torch.complex128: "z",
+           torch.complex64: "c"}.get(LU.dtype)
+    if suf is None:
+        raise TypeError("float64, float32, complex128 or complex64 only")
+    mn, mx = _d(0.0), _d(0.0)
+    _check(getattr(lib(), "tmvx_lu_diag_range_" + suf)(n, LU.data_ptr(), lda, ctypes.byref(mn),
+                                                      ctypes.byref(mx), _stream(stream)),
+           "lu_diag_range")
+    return mn.value, mx.value
 
 
 def logdet(LU, piv, stream=None):

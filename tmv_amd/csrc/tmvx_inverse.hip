@@ -64,6 +64,63 @@ logdet_kernel(i64 n, const T* __restrict__ LU, i64 lda, const i64* __restrict__ 
     }
 }
 
+// out[0] = min_i |U_ii|, out[1] = max_i |U_ii|, with |z| = |re| + |im| for
+// complex entries (CPLX: T is the real type, entries interleaved).
+template <typename T, bool CPLX>
+__global__ void __launch_bounds__(1024)
+diag_range_kernel(i64 n, const T* __restrict__ LU, i64 lda, double* __restrict__ out)
+{
+    __shared__ double s_mn[16], s_mx[16];
+    double mn = INFINITY, mx = 0.0;
+    for (i64 i = threadIdx.x; i < n; i += blockDim.x) {
+        double a;
+        if constexpr (CPLX) {
+            const T* e = LU + 2 * (i + i * lda);
+            a = fabs((double)e[0]) + fabs((double)e[1]);
+        } else {
+            a = fabs((double)LU[i + i * lda]);
+        }
+        mn = fmin(mn, a);
+        mx = fmax(mx, a);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = fmin(mn, __shfl_down(mn, off));
+        mx = fmax(mx, __shfl_down(mx, off));
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { s_mn[wave] = mn; s_mx[wave] = mx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            mn = fmin(mn, s_mn[w]);
+            mx = fmax(mx, s_mx[w]);
+        }
+        out[0] = mn;
+        out[1] = mx;
+    }
+}
+
+template <typename T, bool CPLX>
+int diag_range_impl(i64 n, const T* LU, i64 lda, double* dmin, double* dmax, hipStream_t s)
+{
+    if (n < 0 || lda < (n > 0 ? n : 1) || !dmin || !dmax) return TMVX_ERR_ARG;
+    if (n == 0) { *dmin = INFINITY; *dmax = 0.0; return TMVX_OK; }
+    if (!LU) return TMVX_ERR_ARG;
+    double* d = nullptr;
+    if (hipMalloc((void**)&d, 2 * sizeof(double)) != hipSuccess) return TMVX_ERR_ALLOC;
+    hipLaunchKernelGGL((diag_range_kernel<T, CPLX>), dim3(1), dim3(1024), 0, s, n, LU, lda, d);
+    int rc = hip_status(hipGetLastError());
+    double h[2] = {0.0, 0.0};
+    if (rc == TMVX_OK &&
+        hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = TMVX_ERR_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess && rc == TMVX_OK) rc = TMVX_ERR_HIP;
+    hipFree(d);
+    *dmin = h[0];
+    *dmax = h[1];
+    return rc;
+}
+
 template <typename T>
 int getri_impl(i64 n, const T* LU, i64 lda, const i64* piv, T* X, i64 ldx, hipStream_t s)
 {
@@ -126,5 +183,21 @@ int tmvx_logdet_f64(int64_t n, const double* LU, int64_t lda, const int64_t* piv
 int tmvx_logdet_f32(int64_t n, const float* LU, int64_t lda, const int64_t* piv,
                     double* logdet, double* sign, void* stream)
 { return logdet_impl<float>(n, LU, lda, piv, logdet, sign, (hipStream_t)stream); }
+
+int tmvx_lu_diag_range_f64(int64_t n, const double* LU, int64_t lda, double* dmin, double* dmax,
+                           void* stream)
+{ return diag_range_impl<double, false>(n, LU, lda, dmin, dmax, (hipStream_t)stream); }
+
+int tmvx_lu_diag_range_f32(int64_t n, const float* LU, int64_t lda, double* dmin, double* dmax,
+                           void* stream)
+{ return diag_range_impl<float, false>(n, LU, lda, dmin, dmax, (hipStream_t)stream); }
+
+int tmvx_lu_diag_range_z(int64_t n, const double* LU, int64_t lda, double* dmin, double* dmax,
+                         void* stream)
+{ return diag_range_impl<double, true>(n, LU, lda, dmin, dmax, (hipStream_t)stream); }
+
+int tmvx_lu_diag_range_c(int64_t n, const float* LU, int64_t lda, double* dmin, double* dmax,
+                         void* stream)
+{ return diag_range_impl<float, true>(n, LU, lda, dmin, dmax, (hipStream_t)stream); }
 
 } // extern "C"

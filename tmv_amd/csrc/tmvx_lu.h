@@ -28,6 +28,10 @@ template <typename T>
 int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s,
                int lowprec = TMVX_LOWPREC_F32, int tournament = 0,
                const GetrfNoAlloc* na = nullptr);
+// The same for an m x n matrix: min(m, n) pivots (perm only for m == n).
+template <typename T>
+int getrf_mn_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s,
+                  int lowprec, int tournament, const GetrfNoAlloc* na);
 
 // Tournament pivoting (tmvx_tslu.hip): panel [j0, j0+jb) factored in place
 // on its own columns (selection tree, interchange list piv[j0..j0+jb), the

@@ -36,10 +36,23 @@
 #include <cfloat>
 #include <climits>
 #include <mutex>
+#include <vector>
 
 namespace tmvx {
 
 constexpr int PANEL_NB = 64;        // panel width (columns)
+// FP32 panels may be 128 columns wide (-DTMVX_F32_NB=128: 212 VGPRs, two
+// waves per SIMD, no spills), which halves the launches on the critical
+// path; measured on MI355X (FP32 LU, n = 16384): 99.9 ms against 87.8 ms
+// with 64 columns -- each column step of the wider panel costs more than
+// the launches save -- so 64 is the default.
+#ifndef TMVX_F32_NB
+#define TMVX_F32_NB 64
+#endif
+// Outer panels are NBO_MUL panels wide (trailing updates with K = NBO_MUL NB).
+#ifndef TMVX_NBO_MUL
+#define TMVX_NBO_MUL 4
+#endif
 constexpr int PANEL_BB = 8;         // columns per delayed-update block
 // Rows per workgroup (one per thread); a build option for timing the
 // choice: -DTMVX_PANEL_THREADS=256.
@@ -152,7 +165,7 @@ __device__ long long g_panel_trace_j0;
 // parity), one barrier, wave 0 reduces them, publishes the workgroup's
 // candidate, receives every workgroup's slot (lane q holds slot q), picks
 // the winner and puts it in LDS; a second barrier hands it to every wave.
-template <typename T, int NB, int BB, int NTH>
+template <typename T, int NB, int BB, int NTH, bool SOLO>
 __global__ void __launch_bounds__(NTH)
 panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb, i64* __restrict__ piv,
              PanelWork wk)
@@ -220,15 +233,17 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb, i64* __restrict_
     // Deferred share of the previous block's rank-BB update: chunk j (the
     // BB columns now at row[j*BB ..]) with the block's multipliers (now at
     // row[NB-BB ..]) and its U12 rows (still in s_rows, columns (j+1)*BB ..).
-    // Applied one chunk per column step while the workgroup waits for the
-    // exchange; all are done before the block's own end.
+    // Applied while the workgroup waits for the exchange, chunk j at column
+    // step (j - 1) mod BB (one chunk per step for NB = 64, up to two for
+    // NB = 128); all are done before the block's own end.
     auto deferred_chunk = [&](int kk) __attribute__((always_inline)) {
-        if (kk >= nchunk || !dfr) return;
+        if (!dfr) return;
         typedef typename Vec16<T>::type VT;
         constexpr int VN = Vec16<T>::n;
+        static_assert(NBLK - 2 <= 2 * BB, "two chunks per step at most");
         static_for<1, NBLK - 1>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
-            if (j != kk + 1) return;                                  // uniform
+            if ((j - 1) % BB != kk || j > nchunk) return;             // uniform
             static_for<0, BB>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
                 const T li = row[NB - BB + i];
@@ -298,7 +313,7 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb, i64* __restrict_
                 lanes_argmax<8>(lane < NW ? s_wv[par][lane] : T(-1), myi, bv, bi);
                 const unsigned long long hw = __ballot(lane < NW && myi == bi);
                 const int ww = hw ? (int)__builtin_ctzll(hw) : 0;
-                if (lane < CS::SIZE) {
+                if (!SOLO && lane < CS::SIZE) {
                     const int e = lane >= CS::ROW ? lane - CS::ROW : 0;
                     const unsigned rowpay = half_bits(s_wrow[par][ww][e / WPV], e % WPV);
                     const unsigned pay = lane < CS::POS ? half_bits(bv, lane)
@@ -338,7 +353,17 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb, i64* __restrict_
                     });
                     return __ballot(valid && !have) == 0ull;
                 };
-                {
+                if constexpr (SOLO) {
+                    // one workgroup holds every row: its candidate is the
+                    // winner, no exchange
+                    if (lane == 0) {
+                        bestv = bv;
+                        besti = bi;
+#pragma unroll
+                        for (int f = 0; f < BW; ++f)
+                            bpay[f] = half_bits(s_wrow[par][ww][f / WPV], f % WPV);
+                    }
+                } else {
                     for (int qb = 0; qb < G && ok; qb += 64) {
                         const int q = qb + lane;
                         const bool valid = q < G;
@@ -446,8 +471,9 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb, i64* __restrict_
             const int bpar = kb & 1;
             const unsigned long long btag = (unsigned long long)(j0 + c0 + 1) << 32;
             unsigned long long* bs = wk.bslots + (i64)bpar * BB * RW;
-            for (int e = t; e < BB * RW; e += NTH)
-                if (s_own[e / RW]) cstore(bs + e, btag | s_rw[e]);
+            if (!SOLO)                       // a single workgroup owns every pivot row
+                for (int e = t; e < BB * RW; e += NTH)
+                    if (s_own[e / RW]) cstore(bs + e, btag | s_rw[e]);
             // words held elsewhere: all of a thread's loads in flight at once
             constexpr int RPT = (BB * RW + NTH - 1) / NTH;
             unsigned pend = 0;
@@ -555,34 +581,55 @@ struct PanelPerm {
 template <int NB>
 __device__ void panel_perm_wave(PanelPerm<NB>& P, const i64* lpiv, i64 j0, int jb, i64 nrows)
 {
-    static_assert(NB == 64, "one lane per panel column");
+    static_assert(NB == 64 || NB == 128, "one or two panel columns per lane");
+    constexpr int NPL = NB / 64;
     const int lane = threadIdx.x & 63;
-    long long src = j0 + lane, odst = -1, osrc = -1;
+    long long src[NPL], odst[NPL], osrc[NPL];
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) { src[q] = j0 + lane + 64 * q; odst[q] = -1; osrc[q] = -1; }
+    // entry x of a per-lane array (x uniform): lane x & 63 of chunk x >> 6
+    auto get = [&](const long long* v, int x) -> long long {
+        return (NPL > 1 && x >= 64) ? __shfl(v[NPL - 1], x & 63, 64) : __shfl(v[0], x & 63, 64);
+    };
+    auto set = [&](long long* v, int x, long long val) {
+        if (lane == (x & 63)) {
+            if (NPL > 1 && x >= 64) v[NPL - 1] = val;
+            else v[0] = val;
+        }
+    };
     int nout = 0;
     for (int k = 0; k < jb; ++k) {
         const i64 p = lpiv[k];
         if (p <= j0 + k || p >= nrows) continue;            // uniform
         if (p < j0 + jb) {
             const int kp = (int)(p - j0);
-            const long long a = __shfl(src, k, 64), b = __shfl(src, kp, 64);
-            if (lane == k) src = b;
-            if (lane == kp) src = a;
+            const long long a = get(src, k), b = get(src, kp);
+            set(src, k, b);
+            set(src, kp, a);
         } else {
-            const unsigned long long hit = __ballot(lane < nout && odst == p);
-            int q;
-            if (hit) q = __ffsll((long long)hit) - 1;
-            else {
-                q = nout++;
-                if (lane == q) { odst = p; osrc = p; }
+            int q = -1;
+#pragma unroll
+            for (int c = NPL - 1; c >= 0; --c) {
+                const unsigned long long hit =
+                    __ballot(lane + 64 * c < nout && odst[c] == (long long)p);
+                if (hit) q = 64 * c + __ffsll((long long)hit) - 1;
             }
-            const long long a = __shfl(src, k, 64), b = __shfl(osrc, q, 64);
-            if (lane == k) src = b;
-            if (lane == q) osrc = a;
+            if (q < 0) {
+                q = nout++;
+                set(odst, q, p);
+                set(osrc, q, p);
+            }
+            const long long a = get(src, k), b = get(osrc, q);
+            set(src, k, b);
+            set(osrc, q, a);
         }
     }
-    P.src[lane] = src;
-    P.odst[lane] = odst;
-    P.osrc[lane] = osrc;
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+        P.src[lane + 64 * q] = src[q];
+        P.odst[lane + 64 * q] = odst[q];
+        P.osrc[lane + 64 * q] = osrc[q];
+    }
     if (lane == 0) P.nout = nout;
 }
 
@@ -595,6 +642,7 @@ laswp_panel_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 a0, i64 a1, i64 b0
                    i64 j0, int jb, const i64* __restrict__ piv, i64* __restrict__ perm,
                    int cols_per_wave)
 {
+    constexpr int NPL = NB / 64;
     __shared__ PanelPerm<NB> P;
     __shared__ i64 lpiv[NB];
     if (threadIdx.x < jb) lpiv[threadIdx.x] = piv[j0 + threadIdx.x];
@@ -604,26 +652,42 @@ laswp_panel_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 a0, i64 a1, i64 b0
     const int lane = threadIdx.x & 63;
     const int nout = P.nout;
     if (perm != nullptr && blockIdx.x == 0 && threadIdx.x < 64) {
-        i64 v1 = 0, v2 = 0;
-        if (lane < jb) v1 = perm[P.src[lane]];
-        if (lane < nout) v2 = perm[P.osrc[lane]];
+        i64 v1[NPL], v2[NPL];
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) {
+            const int x = lane + 64 * q;
+            v1[q] = x < jb ? perm[P.src[x]] : 0;
+            v2[q] = x < nout ? perm[P.osrc[x]] : 0;
+        }
         __builtin_amdgcn_s_waitcnt(0);     // every load returned before any store
-        if (lane < jb) perm[j0 + lane] = v1;
-        if (lane < nout) perm[P.odst[lane]] = v2;
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) {
+            const int x = lane + 64 * q;
+            if (x < jb) perm[j0 + x] = v1[q];
+            if (x < nout) perm[P.odst[x]] = v2[q];
+        }
     }
     const i64 nleft = a1 - a0, nright = b1 - b0, total = nleft + nright;
     const i64 wave = (i64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    for (int q = 0; q < cols_per_wave; ++q) {
-        const i64 idx = wave * cols_per_wave + q;
+    for (int cq = 0; cq < cols_per_wave; ++cq) {
+        const i64 idx = wave * cols_per_wave + cq;
         if (idx >= total) break;
         const i64 col = idx < nleft ? a0 + idx : b0 + (idx - nleft);
         T* a = A + col * lda;
-        T v1 = T(0), v2 = T(0);
-        if (lane < jb) v1 = a[P.src[lane]];
-        if (lane < nout) v2 = a[P.osrc[lane]];
+        T v1[NPL], v2[NPL];
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) {
+            const int x = lane + 64 * q;
+            v1[q] = x < jb ? a[P.src[x]] : T(0);
+            v2[q] = x < nout ? a[P.osrc[x]] : T(0);
+        }
         __builtin_amdgcn_s_waitcnt(0);     // every load returned before any store
-        if (lane < jb) a[j0 + lane] = v1;
-        if (lane < nout) a[P.odst[lane]] = v2;
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) {
+            const int x = lane + 64 * q;
+            if (x < jb) a[j0 + x] = v1[q];
+            if (x < nout) a[P.odst[x]] = v2[q];
+        }
     }
 }
 
@@ -631,32 +695,49 @@ laswp_panel_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 a0, i64 a1, i64 b0
 // A12 = A[j0:j0+jb, c0 : c0+ncols].  Lane i of a wave owns row i:
 // it keeps row i of L11 in registers and, column by column, forward-
 // substitutes with x_k broadcast by readlane.  Each wave solves CW columns
-// at once (independent chains interleave) and walks `cpw` columns.
+// at once (independent chains interleave) and walks `cpw` columns.  A
+// 128-row panel is two 64-row halves: solve the top, subtract its product
+// with the lower-left block from the bottom, solve the bottom (lane i then
+// owns rows i and 64 + i).
 constexpr int U12_CW = 4;
 
 template <typename T, int NB>
 __global__ void __launch_bounds__(256)
 trsm_u12_kernel(T* __restrict__ A, i64 lda, i64 j0, int jb, i64 cbeg, i64 ncols, int cpw)
 {
-    static_assert(NB == 64, "one lane per panel row");
+    static_assert(NB == 64 || NB == 128, "one or two panel rows per lane");
     const int lane = threadIdx.x & 63;
     const i64 wave = (i64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const i64 c0 = wave * cpw;
     if (c0 >= ncols) return;
     const i64 c1 = c0 + cpw < ncols ? c0 + cpw : ncols;
-    T Lrow[NB];
+    const int jb1 = jb < 64 ? jb : 64;           // rows of the top half
+    T Lrow[64];
     const T* arow = A + (j0 + lane) + j0 * lda;
-    static_for<0, NB>([&](auto kc) {
+    static_for<0, 64>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        Lrow[k] = (lane < jb && k < lane) ? arow[(i64)k * lda] : T(0);
+        Lrow[k] = (lane < jb1 && k < lane) ? arow[(i64)k * lda] : T(0);
     });
+    T Lbl[NB == 128 ? 64 : 1], Lbr[NB == 128 ? 64 : 1];
+    const bool two = NB == 128 && jb > 64;
+    const int r2 = 64 + lane;                    // bottom-half row of this lane
+    if constexpr (NB == 128) {
+        const T* brow = A + (j0 + r2) + j0 * lda;
+        static_for<0, 64>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            Lbl[k] = (two && r2 < jb) ? brow[(i64)k * lda] : T(0);
+            Lbr[k] = (two && r2 < jb && k < lane) ? brow[(i64)(64 + k) * lda] : T(0);
+        });
+    }
     T* bcol = A + (j0 + lane) + cbeg * lda;
     for (i64 c = c0; c < c1; c += U12_CW) {
-        T x[U12_CW];
+        T x[U12_CW], y[U12_CW];
 #pragma unroll
-        for (int q = 0; q < U12_CW; ++q)
-            x[q] = (lane < jb && c + q < c1) ? bcol[(c + q) * lda] : T(0);
-        static_for<0, NB - 1>([&](auto kc) {
+        for (int q = 0; q < U12_CW; ++q) {
+            x[q] = (lane < jb1 && c + q < c1) ? bcol[(c + q) * lda] : T(0);
+            y[q] = (two && r2 < jb && c + q < c1) ? bcol[64 + (c + q) * lda] : T(0);
+        }
+        static_for<0, 63>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
 #pragma unroll
             for (int q = 0; q < U12_CW; ++q) {
@@ -664,9 +745,191 @@ trsm_u12_kernel(T* __restrict__ A, i64 lda, i64 j0, int jb, i64 cbeg, i64 ncols,
                 if (lane > k) x[q] -= Lrow[k] * xk;
             }
         });
+        if constexpr (NB == 128) {
+            if (two) {
+                static_for<0, 64>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
 #pragma unroll
-        for (int q = 0; q < U12_CW; ++q)
-            if (lane < jb && c + q < c1) bcol[(c + q) * lda] = x[q];
+                    for (int q = 0; q < U12_CW; ++q) y[q] -= Lbl[k] * readlane(x[q], k);
+                });
+                static_for<0, 63>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+#pragma unroll
+                    for (int q = 0; q < U12_CW; ++q) {
+                        const T yk = readlane(y[q], k);
+                        if (lane > k) y[q] -= Lbr[k] * yk;
+                    }
+                });
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < U12_CW; ++q) {
+            if (lane < jb1 && c + q < c1) bcol[(c + q) * lda] = x[q];
+            if (two && r2 < jb && c + q < c1) bcol[64 + (c + q) * lda] = y[q];
+        }
+    }
+}
+
+// The interchanges piv[Jp .. Jp + rows) of up to SS_MAXP 64-column panels
+// on columns [a0, a1) u [b0, b1), panel by panel; the columns in [b0, b1)
+// then take X := L^-1 X on rows [Jp, Jp + rows), L the unit-lower diagonal
+// block of those rows (the U12 block row, TMV's A01 /= A00.lowerTri(
+// UnitDiag), src/TMV_LUDecompose.cpp:262-264).  One launch replaces the
+// per-panel interchange and U12 launches.  Wave q of each workgroup turns
+// panel q's interchange list into its net row movement (panel_perm_wave);
+// then a wave takes its columns one at a time: all loads of a panel
+// before its stores, lane i keeps row 64 q + i of panel q in registers,
+// and the solve runs block by block, the L rows read from memory (L2) in
+// slices, the solved blocks broadcast to the blocks below through LDS.
+// `perm` (the composed permutation of the mixed-precision solver) moves
+// with the rows.
+constexpr int SS_CW = 1;
+constexpr int SS_MAXP = 4;                      // panels per launch (NBO / NB)
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+swap_solve_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 Jp, int rows,
+                  const i64* __restrict__ piv, i64 a0, i64 a1, i64 b0, i64 b1, int cpw,
+                  i64* __restrict__ perm)
+{
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int np = (rows + 63) >> 6;
+    // solved blocks of this wave's columns, read back as broadcasts by the
+    // blocks below them (a readlane per value would be hoisted en masse)
+    __shared__ T s_x[4][SS_MAXP - 1][64][SS_CW];
+    __shared__ PanelPerm<64> P[SS_MAXP];
+    __shared__ i64 lpiv[SS_MAXP * 64];
+    if ((int)threadIdx.x < rows) lpiv[threadIdx.x] = piv[Jp + threadIdx.x];
+    __syncthreads();
+    if (wv < np) {
+        const int jbq = rows - 64 * wv < 64 ? rows - 64 * wv : 64;
+        panel_perm_wave<64>(P[wv], lpiv + 64 * wv, Jp + 64 * wv, jbq, nrows);
+    }
+    __syncthreads();
+    // lane i: row 64 q + i of panel q comes from src[q]; the q-th panel's
+    // i-th outside row moves from osrc[q] to odst[q] (i < nout)
+    i64 src[SS_MAXP], osrc[SS_MAXP], odst[SS_MAXP];
+#pragma unroll
+    for (int q = 0; q < SS_MAXP; ++q) {
+        const bool in = q < np && 64 * q + lane < rows;
+        const bool out = q < np && lane < P[q < np ? q : 0].nout;
+        src[q] = in ? P[q].src[lane] : 0;
+        osrc[q] = out ? P[q].osrc[lane] : -1;
+        odst[q] = out ? P[q].odst[lane] : -1;
+    }
+    if (perm != nullptr && blockIdx.x == 0 && threadIdx.x < 64) {
+        static_for<0, SS_MAXP>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if (q >= np) return;                                      // uniform
+            const int i = 64 * q + lane;
+            const i64 v1 = i < rows ? perm[src[q]] : 0;
+            const i64 v2 = osrc[q] >= 0 ? perm[osrc[q]] : 0;
+            __builtin_amdgcn_s_waitcnt(0);           // every load before any store
+            if (i < rows) perm[Jp + i] = v1;
+            if (osrc[q] >= 0) perm[odst[q]] = v2;
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        });
+    }
+    const i64 nleft = a1 - a0, nright = b1 - b0, total = nleft + nright;
+    const i64 wave = (i64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const i64 cbeg = wave * cpw, cend = cbeg + cpw < total ? cbeg + cpw : total;
+    for (i64 c = cbeg, cl; c < cend; c = cl) {
+        // a group of up to SS_CW columns of one range (a group stops at the
+        // end of the left range)
+        const bool solve = c >= nleft;
+        cl = c + SS_CW < cend ? c + SS_CW : cend;
+        if (!solve && cl > nleft) cl = nleft;
+        T* col[SS_CW];
+#pragma unroll
+        for (int w = 0; w < SS_CW; ++w) {
+            const i64 idx = c + w < cl ? c + w : c;
+            col[w] = A + (idx < nleft ? a0 + idx : b0 + (idx - nleft)) * lda;
+        }
+        const int ncw = (int)(cl - c);
+        T x[SS_MAXP][SS_CW];
+        static_for<0, SS_MAXP>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+#pragma unroll
+            for (int w = 0; w < SS_CW; ++w) x[q][w] = T(0);
+            if (q >= np) return;                                      // uniform
+            const int i = 64 * q + lane;
+            T v1[SS_CW], v2[SS_CW];
+#pragma unroll
+            for (int w = 0; w < SS_CW; ++w) {
+                v1[w] = (w < ncw && i < rows) ? col[w][src[q]] : T(0);
+                v2[w] = (w < ncw && osrc[q] >= 0) ? col[w][osrc[q]] : T(0);
+            }
+            __builtin_amdgcn_s_waitcnt(0);           // every load before any store
+#pragma unroll
+            for (int w = 0; w < SS_CW; ++w) {
+                if (w < ncw && i < rows && !solve) col[w][Jp + i] = v1[w];
+                if (w < ncw && osrc[q] >= 0) col[w][odst[q]] = v2[w];
+                x[q][w] = v1[w];
+            }
+            __builtin_amdgcn_s_waitcnt(0);           // stores done before the next panel's loads
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        });
+        if (!solve) continue;                        // (the loop moves c to cl)
+        // X := L^-1 X, block by block (rows 64 q + lane); the empty asm
+        // statements keep each block's L loads inside its own step
+        static_for<0, SS_MAXP>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if (q >= np) return;                                      // uniform
+            const int i = 64 * q + lane;
+            const T* lrow = A + (Jp + (i < rows ? i : 0)) + Jp * lda;
+            // L is read in slices of 16 columns (16 loads in flight); the row
+            // pointer passes through an empty asm statement that takes the
+            // running x as input, so a slice's loads wait for the previous
+            // slice's work instead of all being hoisted to the top
+            constexpr int KS = 16;
+            static_for<0, q>([&](auto qqc) {
+                constexpr int qq = decltype(qqc)::value;
+                static_for<0, 64 / KS>([&](auto sc) {
+                    constexpr int k0 = decltype(sc)::value * KS;
+                    const T* lp = lrow;
+                    __asm__ volatile("" : "+v"(lp) : "v"(x[q][0]));
+                    T L[KS];
+                    static_for<0, KS>([&](auto kc) {
+                        constexpr int k = decltype(kc)::value;
+                        L[k] = i < rows ? lp[(i64)(64 * qq + k0 + k) * lda] : T(0);
+                    });
+                    static_for<0, KS>([&](auto kc) {
+                        constexpr int k = decltype(kc)::value;
+#pragma unroll
+                        for (int w = 0; w < SS_CW; ++w)
+                            x[q][w] -= L[k] * s_x[wv][qq][k0 + k][w];   // broadcast read
+                    });
+                });
+            });
+            static_for<0, 64 / KS>([&](auto sc) {
+                constexpr int k0 = decltype(sc)::value * KS;
+                const T* lp = lrow;
+                __asm__ volatile("" : "+v"(lp) : "v"(x[q][0]));
+                T L[KS];
+                static_for<0, KS>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    L[k] = (i < rows && k0 + k < lane) ? lp[(i64)(64 * q + k0 + k) * lda] : T(0);
+                });
+                static_for<0, KS>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    if constexpr (k0 + k < 63) {
+#pragma unroll
+                        for (int w = 0; w < SS_CW; ++w) {
+                            const T xk = readlane(x[q][w], k0 + k);
+                            if (lane > k0 + k) x[q][w] -= L[k] * xk;
+                        }
+                    }
+                });
+            });
+#pragma unroll
+            for (int w = 0; w < SS_CW; ++w) {
+                if (w < ncw && i < rows) col[w][Jp + i] = x[q][w];
+                if (q + 1 < SS_MAXP) s_x[wv][q][lane][w] = x[q][w];      // for the blocks below
+            }
+            __asm__ volatile("" ::: "memory");
+        });
     }
 }
 
@@ -701,30 +964,62 @@ int zero_bytes(void* p, size_t bytes, hipStream_t s)
     return hip_status(hipGetLastError());
 }
 
-template <typename T>
-int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s, int lowprec,
-               int tournament, const GetrfNoAlloc* na)
+// Panel timer (diagnostics): while enabled, each panel-kernel launch of an
+// allocating factorisation is bracketed by a pair of HIP events recorded on
+// the stream the panel runs on, up to a fixed number of launches.
+namespace {
+std::mutex g_pt_mu;
+std::vector<hipEvent_t> g_pt_ev;           // 2 per launch
+size_t g_pt_used = 0;
+bool g_pt_on = false;
+}
+
+static bool panel_timer_take(hipEvent_t* e0, hipEvent_t* e1)
 {
-    if (n < 0 || lda < (n > 0 ? n : 1) || (!info && !na) || (na && !na->d_info))
+    std::lock_guard<std::mutex> lk(g_pt_mu);
+    if (!g_pt_on || 2 * g_pt_used + 1 >= g_pt_ev.size()) return false;
+    *e0 = g_pt_ev[2 * g_pt_used];
+    *e1 = g_pt_ev[2 * g_pt_used + 1];
+    ++g_pt_used;
+    return true;
+}
+
+// PA = LU of the m x n matrix A (rows [0, m), columns [0, n)): min(m, n)
+// panel steps; the columns past min(m, n) of a wide matrix take the
+// interchanges and the U solve, the rows past it of a tall one are only
+// eliminated (LAPACK getrf and TMV's RecursiveLUDecompose semantics).
+template <typename T, int NB>
+int getrf_nb_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s,
+                  int lowprec, int tournament, const GetrfNoAlloc* na)
+{
+    if (m < 0 || n < 0 || lda < (m > 0 ? m : 1) || (!info && !na) || (na && !na->d_info))
         return TMVX_ERR_ARG;
     if (info) *info = 0;
-    if (n > 0 && (!A || !piv)) return TMVX_ERR_ARG;
-    if (n == 0) {
+    const i64 R = m < n ? m : n;
+    if (R > 0 && (!A || !piv)) return TMVX_ERR_ARG;
+    if (perm && m != n) return TMVX_ERR_ARG;
+    if (R == 0) {
         if (!na) return TMVX_OK;
-        hipLaunchKernelGGL(getrf_finish_kernel, dim3(1), dim3(64), 0, s, nullptr, nullptr, n,
+        hipLaunchKernelGGL(getrf_finish_kernel, dim3(1), dim3(64), 0, s, nullptr, nullptr, R,
                            na->d_info);
         return hip_status(hipGetLastError());
     }
-    constexpr int NB = PANEL_NB, BB = PANEL_BB, NTH = PANEL_THREADS;
+    constexpr int BB = PANEL_BB, NTH = PANEL_THREADS;
     typedef ColSlot<T, BB> CS;
-    const int Gmax = (int)((n + NTH - 1) / NTH);
-    if (Gmax > PANEL_GMAX) return TMVX_ERR_UNSUPPORTED;   // n > 262144
+    const int Gmax = (int)((m + NTH - 1) / NTH);
+    if ((m + NTH - 1) / NTH > PANEL_GMAX) return TMVX_ERR_UNSUPPORTED;   // m > 262144
 
     const i64 ncslot = 2 * (i64)Gmax * CS::SIZE;
     const i64 nbslot = 2 * (i64)BB * NB * CS::WPV;
     const i64 nslot = ncslot + nbslot;
     const size_t szslots = sizeof(unsigned long long) * (size_t)nslot;
-    const bool tslu = sizeof(T) == 4 && tournament != 0;
+    const bool tslu = sizeof(T) == 4 && tournament != 0 && m == n;
+    // one launch per panel for the interchanges + U12 inside an outer panel
+    // and one per outer panel outside it (64-column panels); wider panels
+    // take the per-panel interchange and U12 kernels
+    const bool fused = NB == 64;
+    constexpr i64 NBO = TMVX_NBO_MUL * NB;
+    static_assert(TMVX_NBO_MUL <= SS_MAXP, "panels per fused launch");
     const size_t szts = tslu ? sizeof(int) * (size_t)tslu_ws_ints(n) : 0;
     const size_t total = szslots + 256 + szts;
     char* ws = nullptr;
@@ -758,9 +1053,9 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
 
     int rc = TMVX_OK;
     {
-        const i64 cnt = n > nslot ? n : nslot;
+        const i64 cnt = R > nslot ? R : nslot;
         hipLaunchKernelGGL(getrf_init_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256),
-                           0, s, wk.cslots, nslot, wk.err, wk.first_zero, n, perm);
+                           0, s, wk.cslots, nslot, wk.err, wk.first_zero, R, perm);
         rc = hip_status(hipGetLastError());
     }
     // Interchanges of panel [j0, j0+jb) on columns [a0,a1) u [b0,b1).
@@ -771,8 +1066,21 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
         const int cpw = 4;
         const i64 waves = (cols + cpw - 1) / cpw;
         const unsigned grid = (unsigned)(waves > 0 ? (waves + 3) / 4 : 1);
-        hipLaunchKernelGGL((laswp_panel_kernel<T, NB>), dim3(grid), dim3(256), 0, st, A, lda, n,
+        hipLaunchKernelGGL((laswp_panel_kernel<T, NB>), dim3(grid), dim3(256), 0, st, A, lda, m,
                            a0, a1, b0, b1, j0, jb, piv, pm, cpw);
+        return hip_status(hipGetLastError());
+    };
+    // Interchanges of the panels in rows [Jp, Jp+rows) on columns [a0,a1) u
+    // [b0,b1), then the U12 solve of those rows on [b0,b1) (fused path).
+    auto swap_solve = [&](i64 Jp, i64 rows, i64 a0, i64 a1, i64 b0, i64 b1, i64* pm,
+                          hipStream_t st) -> int {
+        const i64 cols = (a1 - a0) + (b1 - b0);
+        if (cols <= 0 && !pm) return TMVX_OK;
+        const int cpw = cols > 8192 ? 4 : 1;              // columns per wave
+        const i64 waves = (cols + cpw - 1) / cpw;
+        const unsigned grid = (unsigned)(waves > 0 ? (waves + 3) / 4 : 1);
+        hipLaunchKernelGGL(swap_solve_kernel<T>, dim3(grid), dim3(256), 0, st, A, lda, m, Jp,
+                           (int)rows, piv, a0, a1, b0, b1, cpw, pm);
         return hip_status(hipGetLastError());
     };
     // Rows [j0, j0+jb) of columns [c0, c0+nc) := L(j0..)^-1 times themselves.
@@ -809,36 +1117,47 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
     // few workgroups, latency-bound) then overlaps the big GEMM.  The next
     // panel's interchanges outside it wait for that GEMM (it reads the L
     // and U blocks the interchanges would move).
-    constexpr i64 NBO = 4 * NB;
     SideSet* ss = nullptr;                 // taken from the pool on first need
     hipStream_t side = nullptr;
     hipEvent_t ev_main = nullptr, ev_side = nullptr;
     bool side_pending = false;
-    for (i64 J0 = 0; J0 < n && rc == TMVX_OK; J0 += NBO) {
-        const i64 JE = (n - J0) < NBO ? n : J0 + NBO;      // outer panel [J0, JE)
+    for (i64 J0 = 0; J0 < R && rc == TMVX_OK; J0 += NBO) {
+        const i64 JE = (R - J0) < NBO ? R : J0 + NBO;      // outer panel [J0, JE)
         for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
             const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
             if constexpr (sizeof(T) == 4) {
                 if (tslu) {
                     // tournament pivots, their interchanges on the panel's
                     // own columns and the panel's L and U
-                    if ((rc = tslu_panel<T>(A, lda, n, j0, jb, piv, wk.first_zero, ts_ws, s)) !=
+                    if ((rc = tslu_panel<T>(A, lda, m, j0, jb, piv, wk.first_zero, ts_ws, s)) !=
                         TMVX_OK)
                         break;
                 }
             }
             if (!tslu) {
-                const int G = (int)((n - j0 + NTH - 1) / NTH);
-                hipLaunchKernelGGL((panel_kernel<T, NB, BB, NTH>), dim3(G), dim3(NTH), 0, s,
-                                   A, lda, n, j0, jb, piv, wk);
+                const int G = (int)((m - j0 + NTH - 1) / NTH);
+                hipEvent_t pt0 = nullptr, pt1 = nullptr;
+                const bool timed = !na && panel_timer_take(&pt0, &pt1);
+                if (timed) (void)hipEventRecord(pt0, s);
+                if (G == 1)                    // every row in one workgroup: no exchange
+                    hipLaunchKernelGGL((panel_kernel<T, NB, BB, NTH, true>), dim3(1), dim3(NTH),
+                                       0, s, A, lda, m, j0, jb, piv, wk);
+                else
+                    hipLaunchKernelGGL((panel_kernel<T, NB, BB, NTH, false>), dim3(G), dim3(NTH),
+                                       0, s, A, lda, m, j0, jb, piv, wk);
                 if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
+                if (timed) (void)hipEventRecord(pt1, s);
             }
             // inside the outer panel only; the rest is deferred
-            if ((rc = swap_cols(j0, jb, J0, j0, j0 + jb, JE, perm, s)) != TMVX_OK) break;
             const i64 nr = JE - j0 - jb;
+            if (fused) {
+                if ((rc = swap_solve(j0, jb, J0, j0, j0 + jb, JE, perm, s)) != TMVX_OK) break;
+            } else if ((rc = swap_cols(j0, jb, J0, j0, j0 + jb, JE, perm, s)) != TMVX_OK) {
+                break;
+            }
             if (nr > 0) {
-                if ((rc = solve_rows(j0, jb, j0 + jb, nr, s)) != TMVX_OK) break;
-                rc = gemm_launch<T>(0, 0, n - j0 - jb, nr, jb, T(-1), A + (j0 + jb) + j0 * lda,
+                if (!fused && (rc = solve_rows(j0, jb, j0 + jb, nr, s)) != TMVX_OK) break;
+                rc = gemm_launch<T>(0, 0, m - j0 - jb, nr, jb, T(-1), A + (j0 + jb) + j0 * lda,
                                     lda, A + j0 + (j0 + jb) * lda, lda, T(1),
                                     A + (j0 + jb) + (j0 + jb) * lda, lda, s);
             }
@@ -855,19 +1174,29 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
         // a side stream to come, the caller's stream swaps only the next
         // outer panel's columns; the side stream swaps the rest (the left
         // L blocks and the far trailing columns) before its own work.
-        const i64 nr = n - JE;
+        const i64 nr = n - JE;                              // columns right of the panel
+        const i64 nrow = m - JE;                            // rows below it
         const i64 JE2 = (n - JE) < NBO ? n : JE + NBO;      // next outer panel [JE, JE2)
         const bool has_side = nr > 0 && JE2 < n;
-        for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
-            const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
-            rc = has_side ? swap_cols(j0, jb, JE, JE2, JE2, JE2, nullptr, s)
-                          : swap_cols(j0, jb, 0, J0, JE, n, nullptr, s);
+        if (fused) {
+            // one launch: the outer panel's interchanges and its U12 block
+            // row on the next outer panel's columns (and, without a side
+            // stream to come, the interchanges on the left blocks)
+            rc = swap_solve(J0, JE - J0, 0, has_side ? 0 : J0, JE, JE2, nullptr, s);
+            if (nr <= 0 || rc != TMVX_OK) continue;
+        } else {
+            for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
+                const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
+                rc = has_side ? swap_cols(j0, jb, JE, JE2, JE2, JE2, nullptr, s)
+                              : swap_cols(j0, jb, 0, J0, JE, n, nullptr, s);
+            }
+            if (nr <= 0 || rc != TMVX_OK) continue;
+            // next panel's columns on the caller's stream
+            if ((rc = block_row_u12(J0, JE, JE, JE2, s)) != TMVX_OK) break;
         }
-        if (nr <= 0 || rc != TMVX_OK) continue;
-        // next panel's columns on the caller's stream
-        if ((rc = block_row_u12(J0, JE, JE, JE2, s)) != TMVX_OK) break;
-        rc = trailing(nr, JE2 - JE, JE - J0, A + JE + J0 * lda, A + J0 + JE * lda,
-                      A + JE + JE * lda, bmax_main, s);
+        if (nrow > 0)
+            rc = trailing(nrow, JE2 - JE, JE - J0, A + JE + J0 * lda, A + J0 + JE * lda,
+                          A + JE + JE * lda, bmax_main, s);
         if (rc != TMVX_OK || JE2 >= n) continue;
         // the rest of the trailing matrix on the side stream (without
         // allocation: the same kernels in the same order on s itself)
@@ -886,21 +1215,26 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
             rc = TMVX_ERR_HIP;
             break;
         }
-        for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
-            const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
-            rc = swap_cols(j0, jb, 0, J0, JE2, n, nullptr, side);
+        if (fused) {
+            if ((rc = swap_solve(J0, JE - J0, 0, J0, JE2, n, nullptr, side)) != TMVX_OK) break;
+        } else {
+            for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
+                const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
+                rc = swap_cols(j0, jb, 0, J0, JE2, n, nullptr, side);
+            }
+            if (rc != TMVX_OK) break;
+            if ((rc = block_row_u12(J0, JE, JE2, n, side)) != TMVX_OK) break;
         }
-        if (rc != TMVX_OK) break;
-        if ((rc = block_row_u12(J0, JE, JE2, n, side)) != TMVX_OK) break;
-        rc = trailing(nr, n - JE2, JE - J0, A + JE + J0 * lda, A + J0 + JE2 * lda,
-                      A + JE + JE2 * lda, bmax_side, side);
+        if (nrow > 0)
+            rc = trailing(nrow, n - JE2, JE - J0, A + JE + J0 * lda, A + J0 + JE2 * lda,
+                          A + JE + JE2 * lda, bmax_side, side);
         if (rc != TMVX_OK) break;
         if (side != s && hipEventRecord(ev_side, side) != hipSuccess) { rc = TMVX_ERR_HIP; break; }
         side_pending = true;
     }
     if (na) {                              // asynchronous: info stays on the device
         if (rc != TMVX_OK) return rc;
-        hipLaunchKernelGGL(getrf_finish_kernel, dim3(1), dim3(64), 0, s, wk.err, wk.first_zero, n,
+        hipLaunchKernelGGL(getrf_finish_kernel, dim3(1), dim3(64), 0, s, wk.err, wk.first_zero, R,
                            na->d_info);
         return hip_status(hipGetLastError());
     }
@@ -911,7 +1245,7 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
             rc = rc == TMVX_OK ? TMVX_ERR_HIP : rc;
     }
     int herr = 0;
-    i64 fz = n + 1;
+    i64 fz = R + 1;
     if (hipMemcpyAsync(&herr, wk.err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipMemcpyAsync(&fz, wk.first_zero, sizeof(i64), hipMemcpyDeviceToHost, s) != hipSuccess)
         rc = TMVX_ERR_HIP;
@@ -921,8 +1255,28 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
     if (!sync_ok) return TMVX_ERR_HIP;
     if (rc != TMVX_OK) return rc;
     if (herr) return TMVX_ERR_TIMEOUT;
-    *info = fz <= n ? fz : 0;
+    *info = fz <= R ? fz : 0;
     return TMVX_OK;
+}
+
+template <typename T>
+int getrf_mn_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s,
+                  int lowprec, int tournament, const GetrfNoAlloc* na)
+{
+    // tournament panels (tmvx_tslu.hip) are 64 columns wide
+    if constexpr (sizeof(T) == 4 && TMVX_F32_NB != PANEL_NB) {
+        if (!(tournament != 0 && m == n))
+            return getrf_nb_impl<T, TMVX_F32_NB>(m, n, A, lda, piv, perm, info, s, lowprec,
+                                                 tournament, na);
+    }
+    return getrf_nb_impl<T, PANEL_NB>(m, n, A, lda, piv, perm, info, s, lowprec, tournament, na);
+}
+
+template <typename T>
+int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s, int lowprec,
+               int tournament, const GetrfNoAlloc* na)
+{
+    return getrf_mn_impl<T>(n, n, A, lda, piv, perm, info, s, lowprec, tournament, na);
 }
 
 // Interchange list -> composed permutation.  Step k swaps positions k and
@@ -1010,6 +1364,10 @@ int piv_to_perm(i64 n, const i64* piv, i64* perm, hipStream_t s)
     return hip_status(hipGetLastError());
 }
 
+template int getrf_mn_impl<double>(i64, i64, double*, i64, i64*, i64*, i64*, hipStream_t, int,
+                                   int, const GetrfNoAlloc*);
+template int getrf_mn_impl<float>(i64, i64, float*, i64, i64*, i64*, i64*, hipStream_t, int, int,
+                                  const GetrfNoAlloc*);
 template int getrf_impl<double>(i64, double*, i64, i64*, i64*, i64*, hipStream_t, int, int,
                                 const GetrfNoAlloc*);
 template int getrf_impl<float>(i64, float*, i64, i64*, i64*, i64*, hipStream_t, int, int,
@@ -1045,6 +1403,49 @@ static int scratch_address(void** out)
 
 } // namespace tmvx
 
+extern "C" int tmvx_panel_timer_start(int cap)
+{
+    using namespace tmvx;
+    if (cap <= 0 || cap > (1 << 20)) return TMVX_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_pt_mu);
+    if (g_pt_on) return TMVX_ERR_ARG;
+    g_pt_ev.assign(2 * (size_t)cap, nullptr);
+    for (auto& e : g_pt_ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            for (auto& f : g_pt_ev)
+                if (f) (void)hipEventDestroy(f);
+            g_pt_ev.clear();
+            return TMVX_ERR_HIP;
+        }
+    }
+    g_pt_used = 0;
+    g_pt_on = true;
+    return TMVX_OK;
+}
+
+extern "C" int tmvx_panel_timer_stop(int64_t* launches, double* total_ms)
+{
+    using namespace tmvx;
+    std::lock_guard<std::mutex> lk(g_pt_mu);
+    if (!g_pt_on || !launches || !total_ms) return TMVX_ERR_ARG;
+    int rc = TMVX_OK;
+    double sum = 0.0;
+    for (size_t i = 0; i < g_pt_used && rc == TMVX_OK; ++i) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(g_pt_ev[2 * i + 1]) != hipSuccess ||
+            hipEventElapsedTime(&ms, g_pt_ev[2 * i], g_pt_ev[2 * i + 1]) != hipSuccess)
+            rc = TMVX_ERR_HIP;
+        sum += ms;
+    }
+    *launches = (int64_t)g_pt_used;
+    *total_ms = sum;
+    for (auto& e : g_pt_ev) (void)hipEventDestroy(e);
+    g_pt_ev.clear();
+    g_pt_used = 0;
+    g_pt_on = false;
+    return rc;
+}
+
 #ifdef TMVX_PANEL_TRACE
 extern "C" int tmvx_panel_trace_set(unsigned long long* buf, long long j0)
 {
@@ -1074,6 +1475,20 @@ int tmvx_getrf_f64_noalloc(int64_t n, double* A, int64_t lda, int64_t* piv, int6
     const tmvx::GetrfNoAlloc na{ws, tmvx::NOALLOC_SCRATCH_BYTES, d_info};
     return tmvx::getrf_impl<double>(n, A, lda, piv, nullptr, nullptr, (hipStream_t)stream,
                                     TMVX_LOWPREC_F32, 0, &na);
+}
+
+int tmvx_getrf_mn_f64(int64_t m, int64_t n, double* A, int64_t lda, int64_t* piv, int64_t* info,
+                      void* stream)
+{
+    return tmvx::getrf_mn_impl<double>(m, n, A, lda, piv, nullptr, info, (hipStream_t)stream,
+                                       TMVX_LOWPREC_F32, 0, nullptr);
+}
+
+int tmvx_getrf_mn_f32(int64_t m, int64_t n, float* A, int64_t lda, int64_t* piv, int64_t* info,
+                      void* stream)
+{
+    return tmvx::getrf_mn_impl<float>(m, n, A, lda, piv, nullptr, info, (hipStream_t)stream,
+                                      TMVX_LOWPREC_F32, 0, nullptr);
 }
 
 int tmvx_getrf_f32_tournament(int64_t n, float* A, int64_t lda, int64_t* piv, int64_t* info,

@@ -1,9 +1,12 @@
 """GEMM tile-configuration sweep (the tuning tool).
 
     python -m tmv_amd.tune [--n 8192] [--dtype f64|f32|both] [--reps 3]
+    python -m tmv_amd.tune --getrf 16384 [--dtype f32] [--reps 2]
 
 Times every configuration of the tile table (include/tmvx_tune.h) on each
 operand layout with HIP events and prints TFLOP/s; --json writes the table.
+--getrf times the blocked LU (uniform[-1,1] matrix of that order) instead,
+for profiling its kernels (panel, interchanges, trailing updates).
 """
 from __future__ import annotations
 
@@ -93,7 +96,27 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="instead: time a plain device-to-device copy of this size "
                          "(counter calibration)")
+    ap.add_argument("--getrf", type=int, default=0,
+                    help="instead: time the LU of a uniform matrix of this order")
     a = ap.parse_args()
+    if a.getrf:
+        import time
+        n = a.getrf
+        A0 = torch.empty(n * n, dtype=torch.float64, device="cuda")
+        tmv_amd.fill_uniform(A0, 3)
+        for suf in (("f64", "f32") if a.dtype == "both" else (a.dtype,)):
+            dt = torch.float64 if suf == "f64" else torch.float32
+            A = A0.to(dt).view(n, n).t()
+            for r in range(a.reps):
+                M = A.clone()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                _, info = tmv_amd.getrf(M)
+                torch.cuda.synchronize()
+                print(f"getrf {suf} n={n} rep {r}: {1e3 * (time.perf_counter() - t0):.2f} ms "
+                      f"info={info}", flush=True)
+                del M
+        return
     if a.copy_mib:
         nb = a.copy_mib * (1 << 20)
         src = torch.empty(nb // 8, dtype=torch.float64, device="cuda").fill_(1.0)
