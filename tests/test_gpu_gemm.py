@@ -126,3 +126,31 @@ def test_dgemm_8192_checksums(dev, layout):
     rhs = (e @ Ah) @ Bh
     bound = 4 * n * EPS64 * ((e @ np.abs(Ah)) @ np.abs(Bh))
     assert np.all(np.abs(lhs - rhs) <= bound)
+
+
+@pytest.mark.parametrize("ar,br", [(0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_gemm_large_rowmajor_operands(dev, ar, br, dtype):
+    """Products past 1024^3 with a row-major operand copy that operand to
+    column-major first: the result equals the column-major product of the
+    same values bit for bit (same MFMA arithmetic), with alpha and beta, on
+    ragged sizes; and it meets the k eps |A||B| bound."""
+    m, n, k = 1031, 1043, 1057
+    print(f"gemm large {dtype} layout={ar}{br}", flush=True)
+    A, B, Ad, Bd = _operands(m, n, k, ar, br, dtype)
+    C0 = rng.uniform_matrix(SEED + 23, m, n)
+    C = to_dev(C0, dtype)
+    tmv_amd.gemm(Ad, Bd, C, alpha=-0.75, beta=0.5)
+    Cc = to_dev(C0, dtype)
+    tmv_amd.gemm(to_dev(A, dtype), to_dev(B, dtype), Cc, alpha=-0.75, beta=0.5)
+    torch.cuda.synchronize()
+    assert torch.equal(C, Cc)
+    eps = EPS64 if dtype == torch.float64 else EPS32
+    Af = A.astype(np.float32) if dtype == torch.float32 else A
+    Bf = B.astype(np.float32) if dtype == torch.float32 else B
+    C0f = C0.astype(np.float32) if dtype == torch.float32 else C0
+    Cl = -0.75 * (Af.astype(np.longdouble) @ Bf.astype(np.longdouble)) + 0.5 * C0f
+    mag = 0.75 * (np.abs(Af).astype(np.float64) @ np.abs(Bf).astype(np.float64)) + \
+        0.5 * np.abs(C0f)
+    err = np.abs(to_host(C) - np.asarray(Cl, np.float64))
+    assert (err <= k * eps * mag + 1e-30).all()

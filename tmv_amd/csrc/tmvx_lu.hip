@@ -49,6 +49,11 @@ constexpr int PANEL_NB = 64;        // panel width (columns)
 #ifndef TMVX_F32_NB
 #define TMVX_F32_NB 64
 #endif
+// The next outer panel's U12 on the caller's stream: in the fused launch
+// (1) or on the trsm / GEMM kernels after a fused interchange launch (0).
+#ifndef TMVX_K2_FUSED
+#define TMVX_K2_FUSED 1
+#endif
 // Outer panels are NBO_MUL panels wide (trailing updates with K = NBO_MUL NB).
 #ifndef TMVX_NBO_MUL
 #define TMVX_NBO_MUL 4
@@ -790,7 +795,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 swap_solve_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 Jp, int rows,
                   const i64* __restrict__ piv, i64 a0, i64 a1, i64 b0, i64 b1, int cpw,
-                  i64* __restrict__ perm)
+                  i64* __restrict__ perm, int solve_b)
 {
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -838,9 +843,9 @@ swap_solve_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 Jp, int rows,
     for (i64 c = cbeg, cl; c < cend; c = cl) {
         // a group of up to SS_CW columns of one range (a group stops at the
         // end of the left range)
-        const bool solve = c >= nleft;
+        const bool solve = c >= nleft && solve_b;
         cl = c + SS_CW < cend ? c + SS_CW : cend;
-        if (!solve && cl > nleft) cl = nleft;
+        if (c < nleft && cl > nleft) cl = nleft;
         T* col[SS_CW];
 #pragma unroll
         for (int w = 0; w < SS_CW; ++w) {
@@ -883,7 +888,7 @@ swap_solve_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 Jp, int rows,
             // pointer passes through an empty asm statement that takes the
             // running x as input, so a slice's loads wait for the previous
             // slice's work instead of all being hoisted to the top
-            constexpr int KS = 16;
+            constexpr int KS = sizeof(T) == 4 ? 64 : 32;
             static_for<0, q>([&](auto qqc) {
                 constexpr int qq = decltype(qqc)::value;
                 static_for<0, 64 / KS>([&](auto sc) {
@@ -1072,15 +1077,19 @@ int getrf_nb_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, h
     };
     // Interchanges of the panels in rows [Jp, Jp+rows) on columns [a0,a1) u
     // [b0,b1), then the U12 solve of those rows on [b0,b1) (fused path).
+    // (solve_b = false: interchanges only, on both ranges)
     auto swap_solve = [&](i64 Jp, i64 rows, i64 a0, i64 a1, i64 b0, i64 b1, i64* pm,
-                          hipStream_t st) -> int {
+                          hipStream_t st, bool solve_b = true) -> int {
         const i64 cols = (a1 - a0) + (b1 - b0);
         if (cols <= 0 && !pm) return TMVX_OK;
-        const int cpw = cols > 8192 ? 4 : 1;              // columns per wave
+        // columns per wave: one (the interchanges are a chain of dependent
+        // round trips per column: parallelism hides them), four for a solve
+        // of many columns (its L rows are re-read per column)
+        const int cpw = (solve_b && cols > 8192) ? 4 : 1;
         const i64 waves = (cols + cpw - 1) / cpw;
         const unsigned grid = (unsigned)(waves > 0 ? (waves + 3) / 4 : 1);
         hipLaunchKernelGGL(swap_solve_kernel<T>, dim3(grid), dim3(256), 0, st, A, lda, m, Jp,
-                           (int)rows, piv, a0, a1, b0, b1, cpw, pm);
+                           (int)rows, piv, a0, a1, b0, b1, cpw, pm, solve_b ? 1 : 0);
         return hip_status(hipGetLastError());
     };
     // Rows [j0, j0+jb) of columns [c0, c0+nc) := L(j0..)^-1 times themselves.
@@ -1178,12 +1187,18 @@ int getrf_nb_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, h
         const i64 nrow = m - JE;                            // rows below it
         const i64 JE2 = (n - JE) < NBO ? n : JE + NBO;      // next outer panel [JE, JE2)
         const bool has_side = nr > 0 && JE2 < n;
-        if (fused) {
+        if (fused && TMVX_K2_FUSED) {
             // one launch: the outer panel's interchanges and its U12 block
             // row on the next outer panel's columns (and, without a side
             // stream to come, the interchanges on the left blocks)
             rc = swap_solve(J0, JE - J0, 0, has_side ? 0 : J0, JE, JE2, nullptr, s);
             if (nr <= 0 || rc != TMVX_OK) continue;
+        } else if (fused) {
+            // one launch for the interchanges, then the block row's U12 on
+            // the trsm / GEMM kernels
+            rc = swap_solve(J0, JE - J0, 0, has_side ? 0 : J0, JE, JE2, nullptr, s, false);
+            if (nr <= 0 || rc != TMVX_OK) continue;
+            if ((rc = block_row_u12(J0, JE, JE, JE2, s)) != TMVX_OK) break;
         } else {
             for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
                 const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
@@ -1216,7 +1231,13 @@ int getrf_nb_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, h
             break;
         }
         if (fused) {
-            if ((rc = swap_solve(J0, JE - J0, 0, J0, JE2, n, nullptr, side)) != TMVX_OK) break;
+            // many columns: one launch for the interchanges, the U12 block
+            // row on the trsm (L rows kept in registers across 16 columns)
+            // and GEMM kernels, which move L once per 16 columns instead of
+            // once per column
+            if ((rc = swap_solve(J0, JE - J0, 0, J0, JE2, n, nullptr, side, false)) != TMVX_OK)
+                break;
+            if ((rc = block_row_u12(J0, JE, JE2, n, side)) != TMVX_OK) break;
         } else {
             for (i64 j0 = J0; j0 < JE && rc == TMVX_OK; j0 += NB) {
                 const int jb = (int)((JE - j0) < NB ? (JE - j0) : NB);
