@@ -593,8 +593,14 @@ __device__ void panel_perm_wave(PanelPerm<NB>& P, const i64* lpiv, i64 j0, int j
 #pragma unroll
     for (int q = 0; q < NPL; ++q) { src[q] = j0 + lane + 64 * q; odst[q] = -1; osrc[q] = -1; }
     // entry x of a per-lane array (x uniform): lane x & 63 of chunk x >> 6
+    // (x is uniform: v_readlane, not a cross-lane permute through LDS)
+    auto rl = [](long long v, int l) -> long long {
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, l);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
+        return (long long)(((unsigned long long)hi << 32) | lo);
+    };
     auto get = [&](const long long* v, int x) -> long long {
-        return (NPL > 1 && x >= 64) ? __shfl(v[NPL - 1], x & 63, 64) : __shfl(v[0], x & 63, 64);
+        return (NPL > 1 && x >= 64) ? rl(v[NPL - 1], x & 63) : rl(v[0], x & 63);
     };
     auto set = [&](long long* v, int x, long long val) {
         if (lane == (x & 63)) {
