@@ -55,6 +55,9 @@ constexpr int PANEL_NB = 64;        // panel width (columns)
 #define TMVX_K2_FUSED 1
 #endif
 // Outer panels are NBO_MUL panels wide (trailing updates with K = NBO_MUL NB).
+#ifndef TMVX_NBO_MUL_F64
+#define TMVX_NBO_MUL_F64 4
+#endif
 #ifndef TMVX_NBO_MUL
 #define TMVX_NBO_MUL 4
 #endif
@@ -795,9 +798,7 @@ trsm_u12_kernel(T* __restrict__ A, i64 lda, i64 j0, int jb, i64 cbeg, i64 ncols,
 // `perm` (the composed permutation of the mixed-precision solver) moves
 // with the rows.
 constexpr int SS_CW = 1;
-constexpr int SS_MAXP = 4;                      // panels per launch (NBO / NB)
-
-template <typename T>
+template <typename T, int SS_MAXP>                // SS_MAXP: panels per launch (NBO / NB)
 __global__ void __launch_bounds__(256)
 swap_solve_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 Jp, int rows,
                   const i64* __restrict__ piv, i64 a0, i64 a1, i64 b0, i64 b1, int cpw,
@@ -811,11 +812,11 @@ swap_solve_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 Jp, int rows,
     __shared__ T s_x[4][SS_MAXP - 1][64][SS_CW];
     __shared__ PanelPerm<64> P[SS_MAXP];
     __shared__ i64 lpiv[SS_MAXP * 64];
-    if ((int)threadIdx.x < rows) lpiv[threadIdx.x] = piv[Jp + threadIdx.x];
+    for (int i = threadIdx.x; i < rows; i += blockDim.x) lpiv[i] = piv[Jp + i];
     __syncthreads();
-    if (wv < np) {
-        const int jbq = rows - 64 * wv < 64 ? rows - 64 * wv : 64;
-        panel_perm_wave<64>(P[wv], lpiv + 64 * wv, Jp + 64 * wv, jbq, nrows);
+    for (int q = wv; q < np; q += 4) {
+        const int jbq = rows - 64 * q < 64 ? rows - 64 * q : 64;
+        panel_perm_wave<64>(P[q], lpiv + 64 * q, Jp + 64 * q, jbq, nrows);
     }
     __syncthreads();
     // lane i: row 64 q + i of panel q comes from src[q]; the q-th panel's
@@ -1029,8 +1030,9 @@ int getrf_nb_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, h
     // and one per outer panel outside it (64-column panels); wider panels
     // take the per-panel interchange and U12 kernels
     const bool fused = NB == 64;
-    constexpr i64 NBO = TMVX_NBO_MUL * NB;
-    static_assert(TMVX_NBO_MUL <= SS_MAXP, "panels per fused launch");
+    constexpr int NBOM = sizeof(T) == 8 ? TMVX_NBO_MUL_F64 : TMVX_NBO_MUL;
+    constexpr i64 NBO = NBOM * NB;
+    static_assert(NBOM == 2 || NBOM == 4 || NBOM == 8, "panels per fused launch");
     const size_t szts = tslu ? sizeof(int) * (size_t)tslu_ws_ints(n) : 0;
     const size_t total = szslots + 256 + szts;
     char* ws = nullptr;
@@ -1094,7 +1096,7 @@ int getrf_nb_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, h
         const int cpw = (solve_b && cols > 8192) ? 4 : 1;
         const i64 waves = (cols + cpw - 1) / cpw;
         const unsigned grid = (unsigned)(waves > 0 ? (waves + 3) / 4 : 1);
-        hipLaunchKernelGGL(swap_solve_kernel<T>, dim3(grid), dim3(256), 0, st, A, lda, m, Jp,
+        hipLaunchKernelGGL((swap_solve_kernel<T, NBOM>), dim3(grid), dim3(256), 0, st, A, lda, m, Jp,
                            (int)rows, piv, a0, a1, b0, b1, cpw, pm, solve_b ? 1 : 0);
         return hip_status(hipGetLastError());
     };
