@@ -306,7 +306,8 @@ int tmvx_laswp_f32(int forward, int64_t ncols, float* A, int64_t lda,
  * whose solves are all in the matrix's own precision; the seam it extends
  * is x = b/A with divideUsing(LU), src/TMV_LUDiv.cpp:185-226).
  * A (n x n, FP64, not modified) is factorised in low precision on MFMA
- * (PA = LU with partial pivoting on an FP32 copy; lowprec selects the
+ * (after a random butterfly transformation without interchanges, or with
+ * partial pivoting: tmvx_pivoting below; on an FP32 copy; lowprec selects the
  * arithmetic of the trailing updates: FP32, or FP16 / BF16 with FP32
  * accumulation and the U block row scaled by a power of two per outer
  * panel), then for each of the nrhs columns
@@ -320,7 +321,9 @@ int tmvx_laswp_f32(int forward, int64_t ncols, float* A, int64_t lda,
  * X (n x nrhs, FP64) receives the solution.  Synchronises.  Returns
  * TMVX_ERR_NOT_CONVERGED (with X holding the last iterate and
  * info->backward_error its backward error) when the target is not met.
- * lowprec may carry a tmvx_refine flag.
+ * lowprec may carry a tmvx_refine and a tmvx_pivoting flag.  iterations and
+ * inner_iterations count both attempts when RBT falls back; factor_ms then
+ * runs to the end of the partial-pivoting factorisation.
  */
 enum tmvx_lowprec { TMVX_LOWPREC_F32 = 0, TMVX_LOWPREC_F16 = 1, TMVX_LOWPREC_BF16 = 2 };
 
@@ -335,6 +338,25 @@ enum tmvx_lowprec { TMVX_LOWPREC_F32 = 0, TMVX_LOWPREC_F16 = 1, TMVX_LOWPREC_BF1
  */
 enum tmvx_refine { TMVX_REFINE_GMRES = 16, TMVX_REFINE_CLASSIC = 32 };
 
+/*
+ * Factorisation, or-ed into lowprec.  Default (neither flag) is
+ * TMVX_PIVOT_PARTIAL.
+ *   TMVX_PIVOT_RBT: a random butterfly transformation A' = U^T A V (U, V
+ *     recursive butterflies of depth two; Parker 1995, Baboulin et al.,
+ *     ACM TOMS 39(2), 2013), then A' = LU without interchanges, so no panel
+ *     waits on a grid-wide pivot search; the corrections are
+ *     d = V (LU)^-1 U^T r.  A zero or non-finite pivot, or a refinement that
+ *     stagnates (a step gaining less than 10%) or reaches max_iter, falls
+ *     back to partial pivoting from x0 = 0 (info->pivoting reports it).
+ *     Without interchanges the element growth of dense random matrices
+ *     rises with n (10^3 - 10^4 at n = 500 - 2000), which FP32 cannot
+ *     absorb at large n and kappa: at n = 16384, kappa = 1e4 the attempt
+ *     falls back.  Classical refinement after an RBT hands over to GMRES-IR
+ *     when a step gains less than 2x (unless TMVX_REFINE_CLASSIC is set).
+ *   TMVX_PIVOT_PARTIAL: partial pivoting (TMV's rule, on FP32).
+ */
+enum tmvx_pivoting { TMVX_PIVOT_PARTIAL = 64, TMVX_PIVOT_RBT = 128 };
+
 typedef struct tmvx_ir_info {
     int iterations;            /* refinement steps taken                  */
     int converged;             /* 1 when every column met tol             */
@@ -343,6 +365,9 @@ typedef struct tmvx_ir_info {
     double backward_error;     /* worst column, final iterate             */
     double factor_ms;          /* device time of the factorisation        */
     double refine_ms;          /* device time of all refinement steps     */
+    int pivoting;              /* TMVX_PIVOT_RBT, TMVX_PIVOT_PARTIAL, or  */
+                               /* both: RBT fell back to partial pivoting */
+    int reserved;
 } tmvx_ir_info;
 
 int tmvx_gesv_mixed_f64(int lowprec, int64_t n, int64_t nrhs, const double* A,

@@ -38,6 +38,19 @@ int tmvx_getrf_f32_tournament(int64_t n, float* A, int64_t lda, int64_t* piv,
                               int64_t* info, void* stream);
 
 /*
+ * A = LU without row interchanges (the mixed-precision solver's
+ * factorisation after its random butterfly transformation; not TMV's
+ * path).  piv receives the identity list; *info = the 1-based first zero or
+ * non-finite pivot (the factors are then not usable), else 0.  Arguments as
+ * tmvx_getrf_f64 / _f32.  No counterpart in TMV; exposed for tests and
+ * experiments.
+ */
+int tmvx_getrf_nopiv_f64(int64_t n, double* A, int64_t lda, int64_t* piv,
+                         int64_t* info, void* stream);
+int tmvx_getrf_nopiv_f32(int64_t n, float* A, int64_t lda, int64_t* piv,
+                         int64_t* info, void* stream);
+
+/*
  * Panel timer (diagnostics; no counterpart in TMV).  While started, every
  * panel-kernel launch of the allocating factorisations (tmvx_getrf_*, and
  * through them the solvers) is bracketed by a pair of HIP events recorded

@@ -84,6 +84,8 @@ _SIGS = {
     "tmvx_gemm_batched_f64": (_int, [_i64, _vp, _vp, _vp, _d, _vp, _vp, _d, _vp, _vp]),
     "tmvx_gemm_batched_f32": (_int, [_i64, _vp, _vp, _vp, _f, _vp, _vp, _f, _vp, _vp]),
     "tmvx_getrf_f32_tournament": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
+    "tmvx_getrf_nopiv_f64": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
+    "tmvx_getrf_nopiv_f32": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
     "tmvx_lu_diag_range_f64": (_int, [_i64, _vp, _i64, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
     "tmvx_lu_diag_range_f32": (_int, [_i64, _vp, _i64, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
     "tmvx_lu_diag_range_z": (_int, [_i64, _vp, _i64, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
@@ -442,6 +444,26 @@ def getri_complex(LU, piv, stream=None):
     return X
 
 
+def getrf_nopiv(A, stream=None):
+    """A = LU without row interchanges (the mixed solver's factorisation
+    after its random butterfly transformation).  Returns (piv, info): piv is
+    the identity list, info the 1-based first zero or non-finite pivot (the
+    factors are then unusable) or 0."""
+    torch = _torch()
+    n = _square(A, "A")
+    suf = {torch.float64: "f64", torch.float32: "f32"}.get(A.dtype)
+    if suf is None:
+        raise TypeError("float64 or float32")
+    lda = _colmajor_ld(A, "A")
+    _on_gpu(A)
+    piv = torch.empty(n, dtype=torch.int64, device=A.device)
+    info = _i64(0)
+    _check(getattr(lib(), "tmvx_getrf_nopiv_" + suf)(n, A.data_ptr(), lda, piv.data_ptr(),
+                                                     ctypes.byref(info), _stream(stream)),
+           "getrf_nopiv")
+    return piv, int(info.value)
+
+
 def getrf_tournament(A, stream=None):
     """FP32 PA = LU with tournament pivoting (the mixed solver's panel
     factorisation).  Returns (piv, info) like getrf."""
@@ -585,10 +607,15 @@ class IrInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int),
                 ("factor_info", ctypes.c_int), ("inner_iterations", ctypes.c_int),
                 ("backward_error", ctypes.c_double), ("factor_ms", ctypes.c_double),
-                ("refine_ms", ctypes.c_double)]
+                ("refine_ms", ctypes.c_double), ("pivoting", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        d["pivoting"] = {PIVOT_RBT: "rbt", PIVOT_PARTIAL: "partial",
+                         PIVOT_RBT | PIVOT_PARTIAL: "rbt->partial"}.get(d["pivoting"],
+                                                                       d["pivoting"])
+        return d
 
 
 LOWPREC_F32 = 0
@@ -596,6 +623,9 @@ LOWPREC_F16 = 1
 LOWPREC_BF16 = 2
 REFINE_GMRES = 16      # or-ed into lowprec: GMRES-IR (default for F16 / BF16)
 REFINE_CLASSIC = 32    # or-ed into lowprec: classical refinement (default for F32)
+PIVOT_PARTIAL = 64     # or-ed into lowprec: partial pivoting (the default)
+PIVOT_RBT = 128        # or-ed into lowprec: random butterfly + no interchanges,
+                       # falling back to partial pivoting
 
 
 def gesv_mixed(A, B, X=None, tol=1e-12, max_iter=30, lowprec=LOWPREC_F32, check=True,

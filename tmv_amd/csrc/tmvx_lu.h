@@ -10,8 +10,10 @@ namespace tmvx {
 // A.  Synchronises; *info = 1-based first zero pivot or 0.
 // lowprec (FP32 only): TMVX_LOWPREC_F16 / _BF16 run the trailing updates
 // on FP16 / BF16 MFMA with per-block-row scaling (gemm_lowp_launch).
-// tournament != 0 (FP32 only): tournament pivoting (tmvx_tslu.hip) instead
-// of partial pivoting -- for the mixed-precision solver, not for TMV parity.
+// pivoting: PIVOT_PARTIAL (TMV's rule), PIVOT_TOURNAMENT (FP32 only,
+// tmvx_tslu.hip) or PIVOT_NONE (no interchanges: piv receives the identity
+// list and *info the first zero or non-finite pivot) -- the last two for the
+// mixed-precision solver, not for TMV parity.
 //
 // With na != nullptr (no allocation): the workspace is na->ws (at most
 // na->bytes; larger n is TMVX_ERR_UNSUPPORTED), the look-ahead's side work
@@ -19,6 +21,7 @@ namespace tmvx {
 // nothing is synchronised or copied to the host: na->d_info (device)
 // receives the info, or -1 when a bounded wait expired.  FP32/FP64 partial
 // pivoting only.
+enum { PIVOT_PARTIAL = 0, PIVOT_TOURNAMENT = 1, PIVOT_NONE = 2 };
 struct GetrfNoAlloc {
     void* ws;
     size_t bytes;
@@ -26,12 +29,12 @@ struct GetrfNoAlloc {
 };
 template <typename T>
 int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s,
-               int lowprec = TMVX_LOWPREC_F32, int tournament = 0,
+               int lowprec = TMVX_LOWPREC_F32, int pivoting = PIVOT_PARTIAL,
                const GetrfNoAlloc* na = nullptr);
 // The same for an m x n matrix: min(m, n) pivots (perm only for m == n).
 template <typename T>
 int getrf_mn_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s,
-                  int lowprec, int tournament, const GetrfNoAlloc* na);
+                  int lowprec, int pivoting, const GetrfNoAlloc* na);
 
 // Tournament pivoting (tmvx_tslu.hip): panel [j0, j0+jb) factored in place
 // on its own columns (selection tree, interchange list piv[j0..j0+jb), the

@@ -945,6 +945,103 @@ swap_solve_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 Jp, int rows,
     }
 }
 
+// LU without interchanges (the mixed-precision solver after a random
+// butterfly transformation; never the TMV-parity path).  The panel
+// [j0, j0+jb) is two launches with no grid-wide exchange:
+//
+// genp_diag_kernel: the jb x jb diagonal block, one wave, lane i holding
+// row i in registers.  Step k broadcasts the pivot row by readlane, scales
+// column k below the diagonal by the reciprocal of the pivot and updates the
+// rows below.  A zero or non-finite pivot is recorded in *first_zero
+// (1-based, minimum) and its column is left unscaled and unapplied (the
+// caller treats the factorisation as failed).  piv receives the identity
+// interchange list.
+template <typename T>
+__global__ void __launch_bounds__(64)
+genp_diag_kernel(T* __restrict__ A, i64 lda, i64 j0, int jb, i64* __restrict__ piv,
+                 i64* __restrict__ first_zero)
+{
+    const int lane = threadIdx.x;
+    T row[64];
+    T* ap = A + (j0 + lane) + j0 * lda;
+    static_for<0, 64>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        row[c] = (lane < jb && c < jb) ? ap[(i64)c * lda] : T(0);
+    });
+    i64 bad = 0;
+    static_for<0, 64>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if (k < jb) {                                   // uniform
+            const T p = readlane(row[k], k);
+            const bool ok = p != T(0) && isfinite(p);
+            if (!ok && bad == 0) bad = j0 + k + 1;
+            const T rinv = ok ? T(1) / p : T(0);
+            const T l = row[k] * rinv;
+            if (lane > k && ok) row[k] = l;
+            static_for<k + 1, 64>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const T u = readlane(row[j], k);
+                if (lane > k && ok) row[j] -= l * u;
+            });
+        }
+    });
+    static_for<0, 64>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        if (lane < jb && c < jb) ap[(i64)c * lda] = row[c];
+    });
+    if (lane < jb) piv[j0 + lane] = j0 + lane;
+    if (lane == 0 && bad != 0)
+        atomicMin(reinterpret_cast<unsigned long long*>(first_zero), (unsigned long long)bad);
+}
+
+// genp_l21_kernel: L21 := A21 U11^-1 for the rows [j0 + jb, nrows) below the
+// diagonal block, one row per thread: U11 and the reciprocals of its
+// diagonal in LDS (every lane reads the same word: broadcasts), the row in
+// registers, column-oriented substitution.
+template <typename T>
+__global__ void __launch_bounds__(256)
+genp_l21_kernel(T* __restrict__ A, i64 lda, i64 nrows, i64 j0, int jb)
+{
+    __shared__ __attribute__((aligned(16))) T s_u[64][64];     // s_u[p][k] = U11(p, k)
+    __shared__ T s_r[64];
+    for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
+        const int p = e & 63, k = e >> 6;
+        s_u[p][k] = (p <= k && k < jb) ? A[(j0 + p) + (j0 + k) * lda] : T(0);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int p = threadIdx.x;
+        const T d = s_u[p][p];
+        s_r[p] = (p < jb && d != T(0) && isfinite(d)) ? T(1) / d : T(0);
+    }
+    __syncthreads();
+    const i64 i = j0 + jb + (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    T* ap = A + i + j0 * lda;
+    T x[64];
+    static_for<0, 64>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        x[c] = c < jb ? ap[(i64)c * lda] : T(0);
+    });
+    int off = 0;
+    static_for<0, 64>([&](auto pc) {
+        constexpr int p = decltype(pc)::value;
+        x[p] *= s_r[p];
+        // row p of U11 is read once x[p] is final (an opaque dependency:
+        // otherwise every read is hoisted to the top and the rows spill)
+        __asm__ volatile("" : "+v"(off) : "v"(x[p]));
+        const T* up = &s_u[0][0] + off + p * 64;
+        static_for<p + 1, 64>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            x[k] -= x[p] * up[k];
+        });
+    });
+    static_for<0, 64>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        if (c < jb) ap[(i64)c * lda] = x[c];
+    });
+}
+
 __global__ void getrf_init_kernel(unsigned long long* tags, i64 ntags, int* err,
                                   i64* first_zero, i64 n, i64* perm)
 {
@@ -1002,7 +1099,7 @@ static bool panel_timer_take(hipEvent_t* e0, hipEvent_t* e1)
 // eliminated (LAPACK getrf and TMV's RecursiveLUDecompose semantics).
 template <typename T, int NB>
 int getrf_nb_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s,
-                  int lowprec, int tournament, const GetrfNoAlloc* na)
+                  int lowprec, int pivoting, const GetrfNoAlloc* na)
 {
     if (m < 0 || n < 0 || lda < (m > 0 ? m : 1) || (!info && !na) || (na && !na->d_info))
         return TMVX_ERR_ARG;
@@ -1025,7 +1122,9 @@ int getrf_nb_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, h
     const i64 nbslot = 2 * (i64)BB * NB * CS::WPV;
     const i64 nslot = ncslot + nbslot;
     const size_t szslots = sizeof(unsigned long long) * (size_t)nslot;
-    const bool tslu = sizeof(T) == 4 && tournament != 0 && m == n;
+    const bool tslu = sizeof(T) == 4 && pivoting == PIVOT_TOURNAMENT && m == n;
+    const bool nopiv = pivoting == PIVOT_NONE;
+    if (nopiv && NB != 64) return TMVX_ERR_UNSUPPORTED;
     // one launch per panel for the interchanges + U12 inside an outer panel
     // and one per outer panel outside it (64-column panels); wider panels
     // take the per-panel interchange and U12 kernels
@@ -1037,7 +1136,8 @@ int getrf_nb_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, h
     const size_t total = szslots + 256 + szts;
     char* ws = nullptr;
     if (na) {
-        if (total > na->bytes || tslu || lowprec != TMVX_LOWPREC_F32) return TMVX_ERR_UNSUPPORTED;
+        if (total > na->bytes || tslu || nopiv || lowprec != TMVX_LOWPREC_F32)
+            return TMVX_ERR_UNSUPPORTED;
         ws = static_cast<char*>(na->ws);
     } else if (hipMallocAsync((void**)&ws, total, s) != hipSuccess) {
         return TMVX_ERR_ALLOC;
@@ -1088,6 +1188,11 @@ int getrf_nb_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, h
     // (solve_b = false: interchanges only, on both ranges)
     auto swap_solve = [&](i64 Jp, i64 rows, i64 a0, i64 a1, i64 b0, i64 b1, i64* pm,
                           hipStream_t st, bool solve_b = true) -> int {
+        if (nopiv) {                       // no interchanges: only the solve on [b0, b1)
+            if (!solve_b) return TMVX_OK;
+            a1 = a0;
+            pm = nullptr;
+        }
         const i64 cols = (a1 - a0) + (b1 - b0);
         if (cols <= 0 && !pm) return TMVX_OK;
         // columns per wave: one (the interchanges are a chain of dependent
@@ -1151,7 +1256,19 @@ int getrf_nb_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, h
                         break;
                 }
             }
-            if (!tslu) {
+            if (nopiv) {
+                hipEvent_t pt0 = nullptr, pt1 = nullptr;
+                const bool timed = panel_timer_take(&pt0, &pt1);
+                if (timed) (void)hipEventRecord(pt0, s);
+                hipLaunchKernelGGL((genp_diag_kernel<T>), dim3(1), dim3(64), 0, s, A, lda, j0, jb,
+                                   piv, wk.first_zero);
+                const i64 below = m - j0 - jb;
+                if (below > 0)
+                    hipLaunchKernelGGL((genp_l21_kernel<T>), dim3((unsigned)((below + 255) / 256)),
+                                       dim3(256), 0, s, A, lda, m, j0, jb);
+                if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
+                if (timed) (void)hipEventRecord(pt1, s);
+            } else if (!tslu) {
                 const int G = (int)((m - j0 + NTH - 1) / NTH);
                 hipEvent_t pt0 = nullptr, pt1 = nullptr;
                 const bool timed = !na && panel_timer_take(&pt0, &pt1);
@@ -1290,22 +1407,22 @@ int getrf_nb_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, h
 
 template <typename T>
 int getrf_mn_impl(i64 m, i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s,
-                  int lowprec, int tournament, const GetrfNoAlloc* na)
+                  int lowprec, int pivoting, const GetrfNoAlloc* na)
 {
-    // tournament panels (tmvx_tslu.hip) are 64 columns wide
+    // tournament and no-interchange panels are 64 columns wide
     if constexpr (sizeof(T) == 4 && TMVX_F32_NB != PANEL_NB) {
-        if (!(tournament != 0 && m == n))
+        if (pivoting == PIVOT_PARTIAL)
             return getrf_nb_impl<T, TMVX_F32_NB>(m, n, A, lda, piv, perm, info, s, lowprec,
-                                                 tournament, na);
+                                                 pivoting, na);
     }
-    return getrf_nb_impl<T, PANEL_NB>(m, n, A, lda, piv, perm, info, s, lowprec, tournament, na);
+    return getrf_nb_impl<T, PANEL_NB>(m, n, A, lda, piv, perm, info, s, lowprec, pivoting, na);
 }
 
 template <typename T>
 int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t s, int lowprec,
-               int tournament, const GetrfNoAlloc* na)
+               int pivoting, const GetrfNoAlloc* na)
 {
-    return getrf_mn_impl<T>(n, n, A, lda, piv, perm, info, s, lowprec, tournament, na);
+    return getrf_mn_impl<T>(n, n, A, lda, piv, perm, info, s, lowprec, pivoting, na);
 }
 
 // Interchange list -> composed permutation.  Step k swaps positions k and
@@ -1524,7 +1641,21 @@ int tmvx_getrf_f32_tournament(int64_t n, float* A, int64_t lda, int64_t* piv, in
                               void* stream)
 {
     return tmvx::getrf_impl<float>(n, A, lda, piv, nullptr, info, (hipStream_t)stream,
-                                   TMVX_LOWPREC_F32, 1);
+                                   TMVX_LOWPREC_F32, tmvx::PIVOT_TOURNAMENT);
+}
+
+int tmvx_getrf_nopiv_f64(int64_t n, double* A, int64_t lda, int64_t* piv, int64_t* info,
+                         void* stream)
+{
+    return tmvx::getrf_impl<double>(n, A, lda, piv, nullptr, info, (hipStream_t)stream,
+                                    TMVX_LOWPREC_F32, tmvx::PIVOT_NONE);
+}
+
+int tmvx_getrf_nopiv_f32(int64_t n, float* A, int64_t lda, int64_t* piv, int64_t* info,
+                         void* stream)
+{
+    return tmvx::getrf_impl<float>(n, A, lda, piv, nullptr, info, (hipStream_t)stream,
+                                   TMVX_LOWPREC_F32, tmvx::PIVOT_NONE);
 }
 
 } // extern "C"

@@ -13,6 +13,21 @@
 //               system M^-1 A d = M^-1 r, M = P^T L U the low-precision
 //               factors; x += d.  It converges where classical refinement
 //               does not (16-bit trailing updates, kappa u_low >~ 1).
+//
+// Two factorisations:
+//   partial pivoting (default):  P A = L U on an FP32 copy of A (the blocked
+//               LU of tmvx_lu.hip with TMV's pivot rule);
+//   random butterfly transformation (RBT) + no interchanges, on request (D. S. Parker,
+//               UCLA CSD-950023, 1995; M. Baboulin, J. Dongarra,
+//               J. Herrmann, S. Tomov, ACM TOMS 39(2), 2013, art. 8):
+//               A' = U^T A V with U, V random recursive butterflies of depth
+//               two, then A' = L U without interchanges; A x = b becomes
+//               x = V A'^-1 U^T b.  No panel needs a grid-wide pivot
+//               search, so the factorisation runs at the speed of its
+//               trailing updates.  Refinement against the original A in FP64
+//               removes the effect of the missing pivoting on accuracy; a
+//               breakdown (zero or non-finite pivot), a stagnating or a
+//               non-converging refinement falls back to partial pivoting.
 #include "tmvx_lu.h"
 #include <cmath>
 #include <cstring>
@@ -145,6 +160,194 @@ __global__ void update_kernel(i64 n, i64 nr, double* __restrict__ X, i64 ldx,
     X[i + j * ldx] += s * (double)D32[i + j * n];
 }
 
+// ---- random butterfly transformation --------------------------------
+//
+// A butterfly of order m (even) is B = 2^-1/2 [R0 R1; R0 -R1] with R0, R1
+// random diagonal matrices of order m/2; a recursive butterfly of depth two
+// is W = diag(B1, B2) B, B of order N and B1, B2 of order N/2 (N = n rounded
+// up to a multiple of 4; rows and columns past n are an identity block
+// scaled by `pad`).  W only mixes the four positions k + a N/4 (a = 0..3,
+// k < N/4): restricted to them it is the 4 x 4 matrix
+//   1/2 [ s0 r0a   s1 r0b   s0 r1a   s1 r1b ]
+//       [ s0 r0a  -s1 r0b   s0 r1a  -s1 r1b ]
+//       [ t0 r0a   t1 r0b  -t0 r1a  -t1 r1b ]
+//       [ t0 r0a  -t1 r0b  -t0 r1a   t1 r1b ]
+// with r0a = R0[k], r0b = R0[k + N/4] (r1 likewise from R1), s0, s1 the
+// diagonals of B1 and t0, t1 those of B2 at k.  Every diagonal entry is
+// exp(u / 10), u uniform in [-1/2, 1/2) (Baboulin et al.'s choice), from a
+// SplitMix64 hash of (seed, diagonal, index).  U uses diagonals 0-5, V 6-11.
+__device__ __forceinline__ double rbt_entry(unsigned long long seed, int diag, i64 idx)
+{
+    unsigned long long z =
+        seed + 0x9E3779B97F4A7C15ull * ((((unsigned long long)diag) << 40) + (unsigned long long)idx + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const double u = (double)(z >> 11) * 0x1.0p-53 - 0.5;
+    return exp(0.1 * u);
+}
+
+// W[(set * N4 + k) * 16 + 4 a + b] = the 4 x 4 group matrix of butterfly
+// `set` (0: U, 1: V) at k.
+__global__ void rbt_table_kernel(unsigned long long seed, i64 N4, double* __restrict__ W)
+{
+    const i64 g = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= 2 * N4) return;
+    const int set = g >= N4 ? 1 : 0;
+    const i64 k = g - set * N4;
+    const int d0 = 6 * set;
+    const double r0a = rbt_entry(seed, d0 + 0, k), r0b = rbt_entry(seed, d0 + 0, k + N4);
+    const double r1a = rbt_entry(seed, d0 + 1, k), r1b = rbt_entry(seed, d0 + 1, k + N4);
+    const double s0 = rbt_entry(seed, d0 + 2, k), s1 = rbt_entry(seed, d0 + 3, k);
+    const double t0 = rbt_entry(seed, d0 + 4, k), t1 = rbt_entry(seed, d0 + 5, k);
+    const double w[16] = {s0 * r0a, s1 * r0b,  s0 * r1a,  s1 * r1b,
+                          s0 * r0a, -s1 * r0b, s0 * r1a,  -s1 * r1b,
+                          t0 * r0a, t1 * r0b,  -t0 * r1a, -t1 * r1b,
+                          t0 * r0a, -t1 * r0b, -t0 * r1a, t1 * r1b};
+    double* out = W + g * 16;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) out[e] = 0.5 * w[e];
+}
+
+// Scale of the identity block that pads n up to N: a power of two near the
+// largest |A(i, i)| (else the largest |A(i, 0)|, else 1), so that the block
+// is of the matrix's own magnitude.  One workgroup of 256.
+__global__ void __launch_bounds__(256)
+rbt_pad_kernel(i64 n, const double* __restrict__ A, i64 lda, double* __restrict__ pad)
+{
+    __shared__ double sd[256], sc[256];
+    double d = 0.0, c = 0.0;
+    for (i64 i = threadIdx.x; i < n; i += 256) {
+        d = fmax(d, fabs(A[i + i * lda]));
+        c = fmax(c, fabs(A[i]));
+    }
+    sd[threadIdx.x] = d;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + h]);
+            sc[threadIdx.x] = fmax(sc[threadIdx.x], sc[threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double v = sd[0] > 0.0 ? sd[0] : sc[0];
+        if (!(v > 0.0) || !isfinite(v)) v = 1.0;
+        int e;
+        (void)frexp(v, &e);
+        *pad = ldexp(1.0, e - 1);
+    }
+}
+
+// A32 (N x N) = fl32(U^T A_pad V), and per-(row, chunk) partial sums of |A|
+// for the infinity norm (rowpart[i + chunk n], rows i < n).  Thread k takes
+// the four rows k + a N4 and, for each column group l of its chunk, the 16
+// entries at (k + a N4, l + b N4): one 4 x 4 product on each side.  Loads
+// and stores are coalesced over k; V's group matrix is the same for the
+// whole wave.  grid = (ceil(N4 / 256), nchunk).
+__global__ void __launch_bounds__(256)
+rbt_copy_kernel(i64 n, i64 N4, const double* __restrict__ A, i64 lda,
+                const double* __restrict__ W, const double* __restrict__ pad,
+                float* __restrict__ A32, double* __restrict__ rowpart, int nchunk)
+{
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N4) return;
+    const i64 N = 4 * N4;
+    const i64 cw = (N4 + nchunk - 1) / nchunk;
+    const i64 l0 = (i64)blockIdx.y * cw, l1 = l0 + cw < N4 ? l0 + cw : N4;
+    const double pv = N != n ? *pad : 0.0;
+    double wu[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) wu[e] = W[k * 16 + e];
+    double rs[4] = {0.0, 0.0, 0.0, 0.0};
+    for (i64 l = l0; l < l1; ++l) {
+        const double* wv = W + (N4 + l) * 16;
+        double a[4][4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const i64 j = l + c * N4;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const i64 i = k + r * N4;
+                const bool in = i < n && j < n;
+                a[r][c] = in ? A[i + j * lda] : (i == j ? pv : 0.0);
+                if (in) rs[r] += fabs(a[r][c]);
+            }
+        }
+        double t[4][4];                            // t = Wu^T a
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                t[p][c] = wu[0 * 4 + p] * a[0][c] + wu[1 * 4 + p] * a[1][c] +
+                          wu[2 * 4 + p] * a[2][c] + wu[3 * 4 + p] * a[3][c];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const i64 j = l + q * N4;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                const double o = t[p][0] * wv[0 * 4 + q] + t[p][1] * wv[1 * 4 + q] +
+                                 t[p][2] * wv[2 * 4 + q] + t[p][3] * wv[3 * 4 + q];
+                A32[(k + p * N4) + j * N] = (float)o;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        if (k + r * N4 < n) rowpart[(k + r * N4) + (i64)blockIdx.y * n] = rs[r];
+}
+
+// out(:, j) = U^T [in(:, j) / s_j; 0] (N rows; s_j = max|in(:, j)| from
+// rmax, 1 when rmax is null or the column is zero).  grid = (ceil(N4/256), nr).
+template <typename TO>
+__global__ void rbt_left_kernel(i64 n, i64 N4, i64 nr, const double* __restrict__ in, i64 ldi,
+                                const unsigned long long* __restrict__ rmax,
+                                const double* __restrict__ W, TO* __restrict__ out, i64 ldo)
+{
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    const i64 j = blockIdx.y;
+    if (k >= N4 || j >= nr) return;
+    double s = rmax ? __longlong_as_double((long long)rmax[j]) : 1.0;
+    if (!(s > 0.0)) s = 1.0;
+    double v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = k + r * N4 < n ? in[(k + r * N4) + j * ldi] / s : 0.0;
+    const double* wu = W + k * 16;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+        out[(k + a * N4) + j * ldo] =
+            (TO)(wu[0 * 4 + a] * v[0] + wu[1 * 4 + a] * v[1] + wu[2 * 4 + a] * v[2] +
+                 wu[3 * 4 + a] * v[3]);
+}
+
+// d = (V y(:, j))[0:n]; x(:, j) += s_j d (accumulate) or x(:, j) = d.
+template <typename TI>
+__global__ void rbt_right_kernel(i64 n, i64 N4, i64 nr, const TI* __restrict__ y, i64 ldy,
+                                 const unsigned long long* __restrict__ rmax,
+                                 const double* __restrict__ W, double* __restrict__ x, i64 ldx,
+                                 int accumulate)
+{
+    const i64 l = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    const i64 j = blockIdx.y;
+    if (l >= N4 || j >= nr) return;
+    double s = rmax ? __longlong_as_double((long long)rmax[j]) : 1.0;
+    if (!(s > 0.0)) s = 1.0;
+    double v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = (double)y[(l + c * N4) + j * ldy];
+    const double* wv = W + (N4 + l) * 16;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const i64 i = l + b * N4;
+        if (i >= n) continue;
+        const double d = wv[b * 4 + 0] * v[0] + wv[b * 4 + 1] * v[1] + wv[b * 4 + 2] * v[2] +
+                         wv[b * 4 + 3] * v[3];
+        double* xp = x + i + j * ldx;
+        *xp = accumulate ? *xp + s * d : d;
+    }
+}
+
 __global__ void zero_x_kernel(i64 n, i64 nr, double* __restrict__ X, i64 ldx)
 {
     const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -242,17 +445,41 @@ struct GmresWork {
     int M;                     // restart length (columns of V - 1)
 };
 
+// The butterflies of an RBT factorisation: group matrices W (see
+// rbt_table_kernel), N = 4 N4 >= n, and an N-vector of FP64 scratch.
+struct RbtWork {
+    const double* W;
+    i64 N4;
+    double* tmp;
+};
+
 static int gmres_correction(i64 n, const double* A, i64 lda, const float* LU, const i64* perm,
-                            const double* r, double* x, double tol, int max_inner, GmresWork& g,
-                            int* d_flag, int* inner_done, hipStream_t s)
+                            const RbtWork* rbt, const double* r, double* x, double tol,
+                            int max_inner, GmresWork& g, int* d_flag, int* inner_done,
+                            hipStream_t s)
 {
     const dim3 g1((unsigned)((n + 255) / 256)), b1(256);
     const int M = g.M;
     // z = M^-1 y with FP64 arithmetic on the FP32-stored factors: applied
     // in FP32 the preconditioner's own error (about kappa n u_32) would
     // leave GMRES a perturbed operator and stall it near 1e-4 for BF16
-    // updates at kappa = 1e4.
+    // updates at kappa = 1e4.  After an RBT, M^-1 = V (L U)^-1 U^T.
     auto precond = [&](const double* y, double* z) -> int {
+        if (rbt) {
+            const i64 N = 4 * rbt->N4;
+            const dim3 g4((unsigned)((rbt->N4 + 255) / 256), 1);
+            hipLaunchKernelGGL(rbt_left_kernel<double>, g4, b1, 0, s, n, rbt->N4, (i64)1, y, n,
+                               nullptr, rbt->W, rbt->tmp, N);
+            TMVX_CHECK_LAUNCH();
+            TMVX_TRY((trsv_multi<double, float>(0, 0, 1, N, 1, LU, N, rbt->tmp, N, nullptr,
+                                                rbt->tmp, N, nullptr, d_flag, s)));
+            TMVX_TRY((trsv_multi<double, float>(1, 0, 0, N, 1, LU, N, rbt->tmp, N, nullptr,
+                                                rbt->tmp, N, nullptr, d_flag, s)));
+            hipLaunchKernelGGL(rbt_right_kernel<double>, g4, b1, 0, s, n, rbt->N4, (i64)1,
+                               rbt->tmp, N, nullptr, rbt->W, z, n, 0);
+            TMVX_CHECK_LAUNCH();
+            return TMVX_OK;
+        }
         TMVX_TRY((trsv_multi<double, float>(0, 0, 1, n, 1, LU, n, y, n, perm, z, n, nullptr,
                                             d_flag, s)));
         return trsv_multi<double, float>(1, 0, 0, n, 1, LU, n, z, n, nullptr, z, n, nullptr,
@@ -361,7 +588,7 @@ int gesv_mixed_f32(int lowprec, i64 n, i64 nrhs, const double* A, i64 lda, const
                    hipStream_t s)
 {
     if (!info) return TMVX_ERR_ARG;
-    *info = tmvx_ir_info{0, 0, 0, 0, 0.0, 0.0, 0.0};   // iterations, converged, ...
+    *info = tmvx_ir_info{0, 0, 0, 0, 0.0, 0.0, 0.0, TMVX_PIVOT_PARTIAL, 0};
     if (n < 0 || nrhs < 0 || lda < (n > 0 ? n : 1) || ldb < (n > 0 ? n : 1) ||
         ldx < (n > 0 ? n : 1) || max_iter < 0)
         return TMVX_ERR_ARG;
@@ -370,25 +597,35 @@ int gesv_mixed_f32(int lowprec, i64 n, i64 nrhs, const double* A, i64 lda, const
     if (nrhs > 65535) return TMVX_ERR_UNSUPPORTED;
 
     const int prec = lowprec & 0xf;
-    const bool gmres = (lowprec & TMVX_REFINE_GMRES) != 0 ||
-                       (prec != TMVX_LOWPREC_F32 && (lowprec & TMVX_REFINE_CLASSIC) == 0);
-    const int nchunk = 16;
+    const bool gmres_first = (lowprec & TMVX_REFINE_GMRES) != 0 ||
+                             (prec != TMVX_LOWPREC_F32 && (lowprec & TMVX_REFINE_CLASSIC) == 0);
+    const bool try_rbt = (lowprec & TMVX_PIVOT_RBT) != 0;
+    // after an RBT factorisation, classical refinement that contracts the
+    // error by less than 2x per step hands over to GMRES-IR on the same
+    // factors (unless classical refinement was asked for)
+    const bool may_switch = try_rbt && (lowprec & (TMVX_REFINE_GMRES | TMVX_REFINE_CLASSIC)) == 0;
+    const bool gmres = gmres_first || may_switch;        // GMRES workspace needed
+    const i64 N4 = (n + 3) / 4, N = try_rbt ? 4 * N4 : n;   // RBT order (n padded to 4k)
+    constexpr int NCHUNK = 16;               // column chunks of the residual partials
+    constexpr int RBT_CHUNKS = 32;           // column chunks of the RBT copy
     constexpr int GM = TMVX_GMRES_RESTART;   // GMRES restart length
     constexpr double GTOL = 1e-8;            // inner relative tolerance
-    // Workspace: A32 | piv | perm | R | P | R32 | rowpart | norms | GMRES.
-    const size_t nn = (size_t)n * (size_t)n;
+    // Workspace: A32 | piv | perm | R | P | R32 | rowpart | norms | GMRES | RBT.
+    const size_t nn = (size_t)N * (size_t)N;
     const size_t sz_a32 = sizeof(float) * nn;
-    const size_t sz_piv = sizeof(i64) * (size_t)n;
+    const size_t sz_piv = sizeof(i64) * (size_t)N;
     const size_t sz_r = sizeof(double) * (size_t)n * (size_t)nrhs;
-    const size_t sz_p = sizeof(double) * (size_t)n * (size_t)(nrhs < 16 ? nrhs : 16) * nchunk;
-    const size_t sz_r32 = sizeof(float) * (size_t)n * (size_t)nrhs;
-    const size_t sz_rp = sizeof(double) * (size_t)n * nchunk;
+    const size_t sz_p = sizeof(double) * (size_t)n * (size_t)(nrhs < 16 ? nrhs : 16) * NCHUNK;
+    const size_t sz_r32 = sizeof(float) * (size_t)N * (size_t)nrhs;
+    const size_t sz_rp = sizeof(double) * (size_t)n * (try_rbt ? RBT_CHUNKS : NCHUNK);
     const size_t sz_nrm = sizeof(unsigned long long) * (size_t)(3 * nrhs + 8) + 64;
     const size_t sz_gv = gmres ? sizeof(double) * (size_t)n * (GM + 3) : 0;   // V, w, t
     const size_t sz_gh = gmres ? sizeof(double) * (2 * (GM + 1) + 8) : 0;
+    const size_t sz_w = try_rbt ? sizeof(double) * (size_t)(32 * N4 + 8) : 0;  // W, pad
+    const size_t sz_tmp = try_rbt && gmres ? sizeof(double) * (size_t)N : 0;
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t total = al(sz_a32) + 2 * al(sz_piv) + al(sz_r) + al(sz_p) + al(sz_r32) +
-                         al(sz_rp) + al(sz_nrm) + al(sz_gv) + al(sz_gh);
+                         al(sz_rp) + al(sz_nrm) + al(sz_gv) + al(sz_gh) + al(sz_w) + al(sz_tmp);
     char* ws = nullptr;
     if (hipMallocAsync((void**)&ws, total, s) != hipSuccess) return TMVX_ERR_ALLOC;
     char* q = ws;
@@ -408,9 +645,13 @@ int gesv_mixed_f32(int lowprec, i64 n, i64 nrhs, const double* A, i64 lda, const
         q += al(sz_gv);
         gw.h = reinterpret_cast<double*>(q); q += al(sz_gh);
         gw.P = P;
-        gw.nchunk = nchunk;
+        gw.nchunk = NCHUNK;
         gw.M = GM;
     }
+    double* Wrbt = reinterpret_cast<double*>(q); q += al(sz_w);
+    double* d_pad = try_rbt ? Wrbt + 32 * N4 : nullptr;
+    RbtWork rw{Wrbt, N4, reinterpret_cast<double*>(q)};
+    q += al(sz_tmp);
     unsigned long long* d_anorm = nrm;               // [1]
     unsigned long long* d_bmax = nrm + 8;            // [nrhs]
     unsigned long long* d_rmax = d_bmax + nrhs;      // [nrhs]
@@ -421,85 +662,165 @@ int gesv_mixed_f32(int lowprec, i64 n, i64 nrhs, const double* A, i64 lda, const
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipEventCreate(&e2);
-    int rc = TMVX_OK;
     const dim3 gcol((unsigned)((n + 255) / 256), (unsigned)nrhs);
-    auto chk = [&]() { if (rc == TMVX_OK) rc = hip_status(hipGetLastError()); };
-
-    rc = zero_bytes(nrm, sz_nrm, s);          // norms and the solve flag
-    (void)hipEventRecord(e0, s);
-    // 1. A32 = fl32(A), ||A||_inf, ||b||_inf per column.
-    if (rc == TMVX_OK) {
-        hipLaunchKernelGGL(lowprec_copy_kernel, dim3((unsigned)((n + 255) / 256), nchunk), dim3(256),
-                           0, s, n, A, lda, A32, rowpart, nchunk);
-        chk();
-        hipLaunchKernelGGL(rownorm_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
-                           rowpart, nchunk, d_anorm);
-        chk();
-        hipLaunchKernelGGL(colmax_kernel, gcol, dim3(256), 0, s, n, nrhs, B, ldb, d_bmax);
-        chk();
-    }
-    // 2. FP32 LU with partial pivoting (synchronises).
-    i64 finfo = 0;
-    // partial pivoting (the tournament-pivoted panel of tmvx_tslu.hip is
-    // within 3% of it on this GPU: its local selection costs about as much
-    // per column as the partial-pivoting panel's exchange and local work)
-    if (rc == TMVX_OK) rc = getrf_impl<float>(n, A32, n, piv, perm, &finfo, s, prec, 0);
-    (void)hipEventRecord(e1, s);
-    info->factor_info = (int)finfo;
-    if (rc == TMVX_OK && finfo != 0) rc = TMVX_ERR_SINGULAR;
-
-    // 3. Refinement from x0 = 0 (r0 = b).
+    const dim3 g4col((unsigned)((N4 + 255) / 256), (unsigned)nrhs);
     std::vector<unsigned long long> h(3 * nrhs + 1);
-    double anorm = 0.0, worst = INFINITY;
-    int it = 0;
-    if (rc == TMVX_OK) {
-        hipLaunchKernelGGL(zero_x_kernel, gcol, dim3(256), 0, s, n, nrhs, X, ldx);
-        chk();
-        hipLaunchKernelGGL(residual_reduce_kernel, gcol, dim3(256), 0, s, n, nrhs, B, ldb, P, 0, R);
-        chk();
+    double worst = INFINITY;
+    (void)hipEventRecord(e0, s);
+
+    // One attempt: factorisation (RBT + no interchanges, or partial
+    // pivoting) and refinement from x0 = 0.  *fallback is set when an RBT
+    // attempt breaks down or its refinement stagnates or does not converge.
+    auto attempt = [&](bool rbt, bool* fallback) -> int {
+        int rc = zero_bytes(nrm, sz_nrm, s);          // norms and the solve flag
+        auto chk = [&]() { if (rc == TMVX_OK) rc = hip_status(hipGetLastError()); };
+        // 1. A32 = fl32(A) or fl32(U^T A V), ||A||_inf, ||b||_inf per column.
+        if (rc == TMVX_OK && rbt) {
+            hipLaunchKernelGGL(rbt_table_kernel, dim3((unsigned)((2 * N4 + 255) / 256)), dim3(256),
+                               0, s, 0x5eed5eed0badcafeull, N4, Wrbt);
+            chk();
+            if (N != n) {
+                hipLaunchKernelGGL(rbt_pad_kernel, dim3(1), dim3(256), 0, s, n, A, lda, d_pad);
+                chk();
+            }
+            hipLaunchKernelGGL(rbt_copy_kernel, dim3((unsigned)((N4 + 255) / 256), RBT_CHUNKS),
+                               dim3(256), 0, s, n, N4, A, lda, Wrbt, d_pad, A32, rowpart,
+                               RBT_CHUNKS);
+            chk();
+            hipLaunchKernelGGL(rownorm_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                               n, rowpart, RBT_CHUNKS, d_anorm);
+            chk();
+        } else if (rc == TMVX_OK) {
+            hipLaunchKernelGGL(lowprec_copy_kernel, dim3((unsigned)((n + 255) / 256), NCHUNK),
+                               dim3(256), 0, s, n, A, lda, A32, rowpart, NCHUNK);
+            chk();
+            hipLaunchKernelGGL(rownorm_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                               n, rowpart, NCHUNK, d_anorm);
+            chk();
+        }
+        if (rc == TMVX_OK) {
+            hipLaunchKernelGGL(colmax_kernel, gcol, dim3(256), 0, s, n, nrhs, B, ldb, d_bmax);
+            chk();
+        }
+        // 2. FP32 LU (synchronises).  Partial pivoting keeps the composed
+        // permutation for the solves; the tournament-pivoted panel of
+        // tmvx_tslu.hip is within 3% of partial pivoting on this GPU.
+        i64 finfo = 0;
+        const i64 nf = rbt ? N : n;
+        if (rc == TMVX_OK)
+            rc = getrf_impl<float>(nf, A32, nf, piv, rbt ? nullptr : perm, &finfo, s, prec,
+                                   rbt ? PIVOT_NONE : PIVOT_PARTIAL);
+        (void)hipEventRecord(e1, s);
+        info->factor_info = (int)finfo;
+        if (rc == TMVX_OK && finfo != 0) {
+            if (rbt) { *fallback = true; return TMVX_OK; }
+            return TMVX_ERR_SINGULAR;
+        }
+        // 3. Refinement from x0 = 0 (r0 = b).
+        if (rc == TMVX_OK) {
+            hipLaunchKernelGGL(zero_x_kernel, gcol, dim3(256), 0, s, n, nrhs, X, ldx);
+            chk();
+            hipLaunchKernelGGL(residual_reduce_kernel, gcol, dim3(256), 0, s, n, nrhs, B, ldb, P,
+                               0, R);
+            chk();
+        }
+        double prev = INFINITY;
+        int it = 0;
+        bool use_gmres = gmres_first;
+        while (rc == TMVX_OK) {
+            // max|r|, max|x| per column
+            rc = zero_bytes(d_rmax, sizeof(unsigned long long) * 2 * nrhs, s);
+            if (rc != TMVX_OK) break;
+            hipLaunchKernelGGL(colmax_kernel, gcol, dim3(256), 0, s, n, nrhs, R, n, d_rmax);
+            chk();
+            hipLaunchKernelGGL(colmax_kernel, gcol, dim3(256), 0, s, n, nrhs, X, ldx, d_xmax);
+            chk();
+            if (rc != TMVX_OK) break;
+            if (hipMemcpyAsync(h.data(), d_anorm, sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipMemcpyAsync(h.data() + 1, d_bmax, sizeof(unsigned long long) * 3 * nrhs,
+                               hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                rc = TMVX_ERR_HIP;
+                break;
+            }
+            auto dv = [](unsigned long long b) { double d; memcpy(&d, &b, 8); return d; };
+            const double anorm = dv(h[0]);
+            worst = 0.0;
+            for (i64 j = 0; j < nrhs; ++j) {
+                const double bm = dv(h[1 + j]), rm = dv(h[1 + nrhs + j]),
+                             xm = dv(h[1 + 2 * nrhs + j]);
+                const double den = anorm * xm + bm;
+                const double be = den > 0.0 ? rm / den : (rm > 0.0 ? INFINITY : 0.0);
+                if (!(be <= worst)) worst = be;     // NaN propagates as not converged
+            }
+            if (worst <= tol) { info->converged = 1; break; }
+            // RBT: a classical step gaining less than 2x switches to GMRES-IR
+            // (when allowed); a step gaining less than 10%, or a non-finite
+            // error, hands over to partial pivoting
+            if (rbt && !std::isfinite(worst)) {
+                *fallback = true;
+                break;
+            }
+            if (rbt && it >= 2 && !use_gmres && may_switch && worst > 0.5 * prev) {
+                use_gmres = true;
+            } else if (rbt && it >= 2 && worst > 0.9 * prev) {
+                *fallback = true;
+                break;
+            }
+            prev = worst;
+            if (it >= max_iter) {
+                if (rbt) *fallback = true;
+                break;
+            }
+            if (use_gmres) {
+                // A d = r by preconditioned GMRES, column by column; x += d
+                for (i64 j = 0; j < nrhs && rc == TMVX_OK; ++j)
+                    rc = gmres_correction(n, A, lda, A32, perm, rbt ? &rw : nullptr, R + j * n,
+                                          X + j * ldx, GTOL, GM, gw, d_flag,
+                                          &info->inner_iterations, s);
+                if (rc != TMVX_OK) break;
+            } else if (rbt) {
+                // d = V (LU)^-1 U^T r / s in FP32;  x += s d
+                hipLaunchKernelGGL(rbt_left_kernel<float>, g4col, dim3(256), 0, s, n, N4, nrhs,
+                                   R, n, d_rmax, Wrbt, R32, N);
+                chk();
+                if (rc == TMVX_OK)
+                    rc = getrs_perm<float>(0, N, nrhs, A32, N, nullptr, R32, N, d_flag, s);
+                if (rc != TMVX_OK) break;
+                hipLaunchKernelGGL(rbt_right_kernel<float>, g4col, dim3(256), 0, s, n, N4, nrhs,
+                                   R32, N, d_rmax, Wrbt, X, ldx, 1);
+                chk();
+            } else {
+                // d = (LU)^-1 P r / s in FP32;  x += s d
+                hipLaunchKernelGGL(scale_down_kernel, gcol, dim3(256), 0, s, n, nrhs, R, d_rmax,
+                                   R32);
+                chk();
+                if (rc == TMVX_OK) rc = getrs_perm<float>(0, n, nrhs, A32, n, perm, R32, n, d_flag, s);
+                if (rc != TMVX_OK) break;
+                hipLaunchKernelGGL(update_kernel, gcol, dim3(256), 0, s, n, nrhs, X, ldx, R32,
+                                   d_rmax);
+                chk();
+            }
+            // r = b - A x
+            if (rc == TMVX_OK) rc = launch_residual(n, nrhs, A, lda, X, ldx, B, ldb, P, NCHUNK, R, s);
+            ++it;
+            ++info->iterations;
+        }
+        return rc;
+    };
+
+    int rc = TMVX_OK;
+    bool fallback = false;
+    if (try_rbt) {
+        info->pivoting = TMVX_PIVOT_RBT;
+        rc = attempt(true, &fallback);
+        if (rc == TMVX_OK && fallback) info->pivoting = TMVX_PIVOT_RBT | TMVX_PIVOT_PARTIAL;
     }
-    while (rc == TMVX_OK) {
-        // max|r|, max|x| per column
-        rc = zero_bytes(d_rmax, sizeof(unsigned long long) * 2 * nrhs, s);
-        if (rc != TMVX_OK) break;
-        hipLaunchKernelGGL(colmax_kernel, gcol, dim3(256), 0, s, n, nrhs, R, n, d_rmax);
-        chk();
-        hipLaunchKernelGGL(colmax_kernel, gcol, dim3(256), 0, s, n, nrhs, X, ldx, d_xmax);
-        chk();
-        if (rc != TMVX_OK) break;
-        if (hipMemcpyAsync(h.data(), d_anorm, sizeof(unsigned long long), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipMemcpyAsync(h.data() + 1, d_bmax, sizeof(unsigned long long) * 3 * nrhs,
-                           hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess) { rc = TMVX_ERR_HIP; break; }
-        auto dv = [](unsigned long long b) { double d; memcpy(&d, &b, 8); return d; };
-        anorm = dv(h[0]);
-        worst = 0.0;
-        for (i64 j = 0; j < nrhs; ++j) {
-            const double bm = dv(h[1 + j]), rm = dv(h[1 + nrhs + j]), xm = dv(h[1 + 2 * nrhs + j]);
-            const double den = anorm * xm + bm;
-            const double be = den > 0.0 ? rm / den : (rm > 0.0 ? INFINITY : 0.0);
-            if (!(be <= worst)) worst = be;     // NaN propagates as not converged
-        }
-        if (worst <= tol) { info->converged = 1; break; }
-        if (it >= max_iter) break;
-        if (gmres) {
-            // A d = r by preconditioned GMRES, column by column; x += d
-            for (i64 j = 0; j < nrhs && rc == TMVX_OK; ++j)
-                rc = gmres_correction(n, A, lda, A32, perm, R + j * n, X + j * ldx, GTOL, GM, gw,
-                                      d_flag, &info->inner_iterations, s);
-            if (rc != TMVX_OK) break;
-        } else {
-            // d = (LU)^-1 P r / s in FP32;  x += s d
-            hipLaunchKernelGGL(scale_down_kernel, gcol, dim3(256), 0, s, n, nrhs, R, d_rmax, R32);
-            chk();
-            if (rc == TMVX_OK) rc = getrs_perm<float>(0, n, nrhs, A32, n, perm, R32, n, d_flag, s);
-            if (rc != TMVX_OK) break;
-            hipLaunchKernelGGL(update_kernel, gcol, dim3(256), 0, s, n, nrhs, X, ldx, R32, d_rmax);
-            chk();
-        }
-        // r = b - A x
-        if (rc == TMVX_OK) rc = launch_residual(n, nrhs, A, lda, X, ldx, B, ldb, P, nchunk, R, s);
-        ++it;
+    if (!try_rbt || (rc == TMVX_OK && fallback)) {
+        info->converged = 0;
+        bool unused = false;
+        rc = attempt(false, &unused);
     }
     (void)hipEventRecord(e2, s);
     int hflag = 0;
@@ -515,7 +836,6 @@ int gesv_mixed_f32(int lowprec, i64 n, i64 nrhs, const double* A, i64 lda, const
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(e2);
-    info->iterations = it;
     info->backward_error = worst;
     if (rc == TMVX_OK && !info->converged) rc = TMVX_ERR_NOT_CONVERGED;
     return rc;
@@ -529,9 +849,13 @@ extern "C" int tmvx_gesv_mixed_f64(int lowprec, int64_t n, int64_t nrhs, const d
                                    void* stream)
 {
     const int prec = lowprec & 0xf, flags = lowprec & ~0xf;
+    const int refine = flags & (TMVX_REFINE_GMRES | TMVX_REFINE_CLASSIC);
+    const int pivot = flags & (TMVX_PIVOT_RBT | TMVX_PIVOT_PARTIAL);
     if ((prec != TMVX_LOWPREC_F32 && prec != TMVX_LOWPREC_F16 && prec != TMVX_LOWPREC_BF16) ||
-        (flags & ~(TMVX_REFINE_GMRES | TMVX_REFINE_CLASSIC)) != 0 ||
-        flags == (TMVX_REFINE_GMRES | TMVX_REFINE_CLASSIC))
+        (flags & ~(TMVX_REFINE_GMRES | TMVX_REFINE_CLASSIC | TMVX_PIVOT_RBT |
+                   TMVX_PIVOT_PARTIAL)) != 0 ||
+        refine == (TMVX_REFINE_GMRES | TMVX_REFINE_CLASSIC) ||
+        pivot == (TMVX_PIVOT_RBT | TMVX_PIVOT_PARTIAL))
         return TMVX_ERR_UNSUPPORTED;
     return tmvx::gesv_mixed_f32(lowprec, n, nrhs, A, lda, B, ldb, X, ldx, tol, max_iter, info,
                                 (hipStream_t)stream);
