@@ -502,12 +502,14 @@ def mixed_section(torch, tmv_amd, a):
         b64 = b64.view(64, n).t()                 # column-major n x 64
         b1 = b if nrhs == 1 else b[:, 0].contiguous()
         legs = [("f32", tmv_amd.LOWPREC_F32, 1), ("f32", tmv_amd.LOWPREC_F32, 64),
-                ("f16", tmv_amd.LOWPREC_F16, 1), ("bf16", tmv_amd.LOWPREC_BF16, 1)]
+                ("f16", tmv_amd.LOWPREC_F16, 1), ("bf16", tmv_amd.LOWPREC_BF16, 1),
+                ("f16x2", tmv_amd.LOWPREC_F16X2, 1), ("bf16x2", tmv_amd.LOWPREC_BF16X2, 1)]
+        classic = (tmv_amd.LOWPREC_F32, tmv_amd.LOWPREC_F16X2)   # default refinement
         for kappa, seed in ((1e2, SEED + 14), (a.kappa, None), (1e6, SEED + 18)):
             M = A if seed is None else conditioned_gpu(n, kappa, seed, torch, tmv_amd)
             for name, lp, nr in legs:
                 rhs = b1 if nr == 1 else b64
-                it = 50 if lp == tmv_amd.LOWPREC_F32 else a.lowp_iter
+                it = 50 if lp in classic else a.lowp_iter
                 tmv_amd.gesv_mixed(M, rhs, tol=1e-12, max_iter=it, lowprec=lp, check=False)
                 torch.cuda.synchronize()
                 t1 = time.perf_counter()
@@ -518,7 +520,7 @@ def mixed_section(torch, tmv_amd, a):
                 fl = 2.0 / 3.0 * n ** 3 + 2.0 * n * n * nr
                 grid[f"kappa{kappa:.0e}_{name}_nrhs{nr}"] = {
                     "ms": round(dt * 1e3, 3), "tflops": round(fl / dt / 1e12, 3),
-                    "refine": "classic" if lp == tmv_amd.LOWPREC_F32 else "gmres",
+                    "refine": "classic" if lp in classic else "gmres",
                     "iterations": inf["iterations"], "inner_iterations": inf["inner_iterations"],
                     "converged": inf["converged"], "status": inf["status"],
                     "factor_ms": round(inf["factor_ms"], 3),

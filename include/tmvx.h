@@ -325,11 +325,18 @@ int tmvx_laswp_f32(int forward, int64_t ncols, float* A, int64_t lda,
  * inner_iterations count both attempts when RBT falls back; factor_ms then
  * runs to the end of the partial-pivoting factorisation.
  */
-enum tmvx_lowprec { TMVX_LOWPREC_F32 = 0, TMVX_LOWPREC_F16 = 1, TMVX_LOWPREC_BF16 = 2 };
+enum tmvx_lowprec {
+    TMVX_LOWPREC_F32 = 0, TMVX_LOWPREC_F16 = 1, TMVX_LOWPREC_BF16 = 2,
+    /* split operands on the same 16-bit MFMA: x = hi + 2^-S lo, three
+       products per tile (hi hi + hi lo + lo hi), FP32 accumulation; about
+       22 (FP16, S = 11) / 16 (BF16, S = 8) significant operand bits */
+    TMVX_LOWPREC_F16X2 = 3, TMVX_LOWPREC_BF16X2 = 4
+};
 
 /*
  * Refinement method, or-ed into lowprec.  Default: classical refinement
- * after an FP32 factorisation; GMRES-IR after FP16 / BF16 trailing updates.
+ * after an FP32 or FP16X2 factorisation; GMRES-IR after FP16, BF16 or
+ * BF16X2 trailing updates.
  * GMRES-IR (Carson & Higham's three-precision refinement) solves the
  * correction equation A d = r by GMRES in FP64, left-preconditioned by the
  * low-precision factors (restart 50, relative inner tolerance 1e-8, one
@@ -377,9 +384,10 @@ int tmvx_gesv_mixed_f64(int lowprec, int64_t n, int64_t nrhs, const double* A,
 
 /*
  * C := alpha A B + beta C for FP32 column-major operands computed on FP16
- * or BF16 MFMA (lowprec = TMVX_LOWPREC_F16 / _BF16) with FP32 accumulation:
- * the trailing-update kernel of the low-precision factorisations.  A and B
- * are rounded to the 16-bit format; scale_b != 0 first multiplies B by a
+ * or BF16 MFMA (lowprec = TMVX_LOWPREC_F16 / _BF16, or the split-operand
+ * _F16X2 / _BF16X2) with FP32 accumulation: the trailing-update kernel of
+ * the low-precision factorisations.  A and B are rounded to the 16-bit
+ * format (one or two terms each); scale_b != 0 first multiplies B by a
  * power of two fitted to max |B| (and divides the product by it), so FP16
  * keeps B's leading bits whatever its magnitude.  No counterpart in TMV.
  */

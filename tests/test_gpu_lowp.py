@@ -23,8 +23,14 @@ from tmv_amd import rng
 
 pytestmark = pytest.mark.gpu
 SEED = 0x7A15EED
-U16 = {tmv_amd.LOWPREC_F16: 2.0 ** -11, tmv_amd.LOWPREC_BF16: 2.0 ** -8}
-NAME = {tmv_amd.LOWPREC_F16: "f16", tmv_amd.LOWPREC_BF16: "bf16"}
+# operand error per product: one rounding to 16 bits (u16 each), or, split
+# into hi + 2^-S lo, the rounding of lo (u16^2) plus the dropped lo x lo
+# term (u16^2): 3 u16^2 per product, written as 2 u_eff below with u_eff =
+# 1.5 u16^2
+U16 = {tmv_amd.LOWPREC_F16: 2.0 ** -11, tmv_amd.LOWPREC_BF16: 2.0 ** -8,
+       tmv_amd.LOWPREC_F16X2: 1.5 * 2.0 ** -22, tmv_amd.LOWPREC_BF16X2: 1.5 * 2.0 ** -16}
+NAME = {tmv_amd.LOWPREC_F16: "f16", tmv_amd.LOWPREC_BF16: "bf16",
+        tmv_amd.LOWPREC_F16X2: "f16x2", tmv_amd.LOWPREC_BF16X2: "bf16x2"}
 
 
 def gemm_check(m, n, k, lp, scale_b=True, bscale=1.0, alpha=-1.0, beta=1.0, seed=0):
@@ -44,13 +50,31 @@ def gemm_check(m, n, k, lp, scale_b=True, bscale=1.0, alpha=-1.0, beta=1.0, seed
     return float((err / bound).max()), C
 
 
-@pytest.mark.parametrize("lp", [tmv_amd.LOWPREC_F16, tmv_amd.LOWPREC_BF16])
+@pytest.mark.parametrize("lp", [tmv_amd.LOWPREC_F16, tmv_amd.LOWPREC_BF16,
+                                tmv_amd.LOWPREC_F16X2, tmv_amd.LOWPREC_BF16X2])
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 5), (128, 128, 32), (300, 257, 256),
                                    (1000, 700, 64), (513, 129, 300)])
 def test_gemm_lowp_bound(dev, lp, m, n, k):
     print(f"gemm_lowp {NAME[lp]} {m}x{n}x{k}", flush=True)
     worst, _ = gemm_check(m, n, k, lp, seed=m + n + k)
     assert worst <= 1.0, worst
+
+
+@pytest.mark.parametrize("plain,split", [(tmv_amd.LOWPREC_F16, tmv_amd.LOWPREC_F16X2),
+                                         (tmv_amd.LOWPREC_BF16, tmv_amd.LOWPREC_BF16X2)])
+def test_gemm_lowp_split_gains_bits(dev, plain, split):
+    """The split operands' error is far below the single-term one (a split
+    that dropped its second term would fail this)."""
+    e1, C1 = gemm_check(300, 257, 256, plain, seed=11)
+    e2, C2 = gemm_check(300, 257, 256, split, seed=11)
+    A = rng.uniform_matrix(SEED + 11, 300, 256)
+    B = rng.uniform_matrix(SEED + 12, 256, 257)
+    C0 = rng.uniform_matrix(SEED + 13, 300, 257)
+    ref = -(A.astype(np.float32).astype(np.float64) @ B.astype(np.float32).astype(np.float64)) \
+        + C0.astype(np.float32)
+    d1 = np.abs(C1 - ref).max()
+    d2 = np.abs(C2 - ref).max()
+    assert d2 < d1 / 64, (d1, d2)
 
 
 @pytest.mark.parametrize("bscale", [1e6, 1e-6])
@@ -115,6 +139,27 @@ def test_mixed_lowp_gmres_n4096_kappa1e4(dev):
     solve_check(A, dA, b, F16)
 
 
+@pytest.mark.parametrize("lp", [tmv_amd.LOWPREC_F16X2, tmv_amd.LOWPREC_BF16X2])
+def test_mixed_split_n4096_kappa1e4(dev, lp):
+    """n = 4096, kappa = 1e4 with split-operand 16-bit trailing updates
+    (classical refinement after FP16X2, GMRES-IR after BF16X2 by default)."""
+    n = 4096
+    print(f"mixed {NAME[lp]} n={n} kappa=1e4", flush=True)
+    dA = conditioned_gpu(n, 1e4, SEED + 4097)
+    A = dA.cpu().numpy()
+    b = np.random.default_rng(4097).standard_normal(n)
+    solve_check(A, dA, b, lp, max_iter=30)
+
+
+@pytest.mark.parametrize("lp,n,kappa", [(tmv_amd.LOWPREC_F16X2, 600, 1e6),
+                                        (tmv_amd.LOWPREC_BF16X2, 600, 1e6),
+                                        (tmv_amd.LOWPREC_BF16X2, 1500, 1e4)])
+def test_mixed_split_converges(dev, lp, n, kappa):
+    A = conditioned(n, kappa, SEED + n + int(np.log10(kappa)))
+    b = np.random.default_rng(n).standard_normal(n)
+    solve_check(A, tmv_amd.colmajor(torch.from_numpy(A).cuda()), b, lp, max_iter=30)
+
+
 @pytest.mark.parametrize("lp,kappa", [(BF16, 1e4), (F16, 1e6)])
 def test_mixed_lowp_gmres_stall_reported(dev, lp, kappa):
     """Beyond what GMRES(50) can recover from the 16-bit factors (BF16 at
@@ -177,9 +222,10 @@ def test_mixed_lowp_reports_not_converged(dev):
 
 
 def test_mixed_refine_flags_rejected(dev):
-    """Both refinement flags at once, or an unknown flag: unsupported."""
+    """Both refinement flags at once, an unknown flag bit or an unknown
+    precision: unsupported."""
     A = tmv_amd.colmajor(torch.eye(8, dtype=torch.float64).cuda())
     b = torch.ones(8, dtype=torch.float64).cuda()
-    for lp in (tmv_amd.REFINE_GMRES | tmv_amd.REFINE_CLASSIC, 64, 3):
+    for lp in (tmv_amd.REFINE_GMRES | tmv_amd.REFINE_CLASSIC, 256, 5):
         x, info = tmv_amd.gesv_mixed(A, b, lowprec=lp, check=False)
         assert info["status"] == tmv_amd.ERR_UNSUPPORTED

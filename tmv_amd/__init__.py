@@ -621,7 +621,9 @@ class IrInfo(ctypes.Structure):
 LOWPREC_F32 = 0
 LOWPREC_F16 = 1
 LOWPREC_BF16 = 2
-REFINE_GMRES = 16      # or-ed into lowprec: GMRES-IR (default for F16 / BF16)
+LOWPREC_F16X2 = 3      # split operands (hi + lo) on FP16 MFMA
+LOWPREC_BF16X2 = 4     # split operands (hi + lo) on BF16 MFMA
+REFINE_GMRES = 16      # or-ed into lowprec: GMRES-IR (default for F16 / BF16 / BF16X2)
 REFINE_CLASSIC = 32    # or-ed into lowprec: classical refinement (default for F32)
 PIVOT_PARTIAL = 64     # or-ed into lowprec: partial pivoting (the default)
 PIVOT_RBT = 128        # or-ed into lowprec: random butterfly + no interchanges,

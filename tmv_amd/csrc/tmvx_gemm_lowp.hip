@@ -12,6 +12,16 @@
 //   pivoting, and B is the U12 block row, so one scale per block row is the
 //   per-panel scaling of the factorisation.
 //
+// Split operands (TMVX_LOWPREC_F16X2 / _BF16X2): each FP32 operand x is
+// carried as two 16-bit terms, hi = fl16(x) and lo = fl16((x - hi) 2^S)
+// (x - hi is exact in FP32; S = 11 for FP16, 8 for BF16, so lo keeps the
+// next 11 / 8 bits at hi's magnitude), and the product is
+// hi_a hi_b + 2^-S (hi_a lo_b + lo_a hi_b): three MFMA products per tile
+// into two FP32 accumulators, the lo x lo term dropped.  The operands then
+// carry about 22 (FP16) / 16 (BF16) significant bits instead of 11 / 8
+// (the technique of H. Ootomo and R. Yokota, Int. J. High Perform. Comput.
+// Appl. 36 (2022) 475-491, on 16-bit matrix units).
+//
 // Tile: 128 x 128 per workgroup (4 waves, 2 x 2, 64 x 64 each = 4 x 4 MFMA
 // tiles), K step 32.  A is read with one coalesced scalar load per row per
 // k; B columns are contiguous in k.  Both tiles are stored k-contiguous in
@@ -42,6 +52,18 @@ template <> struct LowpT<TMVX_LOWPREC_BF16> {
     { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 };
 
+// Arithmetic of a lowprec value: the 16-bit MFMA type, the number of terms
+// per operand and the exponent shift of the second term.
+template <int MT> struct LowpMode {
+    static constexpr int BASE = MT, TERMS = 1, SHIFT = 0;
+};
+template <> struct LowpMode<TMVX_LOWPREC_F16X2> {
+    static constexpr int BASE = TMVX_LOWPREC_F16, TERMS = 2, SHIFT = 11;
+};
+template <> struct LowpMode<TMVX_LOWPREC_BF16X2> {
+    static constexpr int BASE = TMVX_LOWPREC_BF16, TERMS = 2, SHIFT = 8;
+};
+
 constexpr int LP_BM = 128, LP_BN = 128, LP_BK = 32, LP_PAD = 8;
 constexpr float LP_TARGET = 16384.0f;       // 2^14
 
@@ -64,10 +86,13 @@ gemm_lowp_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, 
                  const float* __restrict__ B, i64 ldb, float beta, float* __restrict__ C, i64 ldc,
                  const unsigned* __restrict__ bmax)
 {
-    typedef typename LowpT<MT>::type ht;
-    typedef typename LowpT<MT>::vec hv;
-    __shared__ __attribute__((aligned(16))) ht As[LP_BM][LP_BK + LP_PAD];
-    __shared__ __attribute__((aligned(16))) ht Bs[LP_BN][LP_BK + LP_PAD];
+    typedef LowpMode<MT> Mode;
+    typedef LowpT<Mode::BASE> LT;
+    typedef typename LT::type ht;
+    typedef typename LT::vec hv;
+    constexpr int NT = Mode::TERMS;
+    __shared__ __attribute__((aligned(16))) ht As[NT][LP_BM][LP_BK + LP_PAD];
+    __shared__ __attribute__((aligned(16))) ht Bs[NT][LP_BN][LP_BK + LP_PAD];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int li = lane & 15, g = lane >> 4;
     const int wm = wave & 1, wn = wave >> 1;
@@ -79,10 +104,13 @@ gemm_lowp_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, 
     const int lr = t & 127, lk = (t >> 7) * 16;
     const bool b_vec = (ldb % 4) == 0 && ((uintptr_t)B % 16) == 0;
 
-    f4 acc[4][4];
-    static_for<0, 4>([&](auto ic) {
-        static_for<0, 4>([&](auto jc) {
-            acc[decltype(ic)::value][decltype(jc)::value] = f4{0, 0, 0, 0};
+    f4 acc[NT][4][4];          // [0]: hi x hi; [1]: the cross terms, scaled by 2^S
+    static_for<0, NT>([&](auto tc) {
+        static_for<0, 4>([&](auto ic) {
+            static_for<0, 4>([&](auto jc) {
+                acc[decltype(tc)::value][decltype(ic)::value][decltype(jc)::value] =
+                    f4{0, 0, 0, 0};
+            });
         });
     });
     float ra[16], rb[16];
@@ -109,18 +137,32 @@ gemm_lowp_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, 
         }
     };
     auto lstore = [&]() {
-        hv a0, a1, b0, b1;
+        constexpr float up = (float)(1 << Mode::SHIFT);
+        hv a0[NT], a1[NT], b0[NT], b1[NT];
         static_for<0, 8>([&](auto vc) {
             constexpr int v = decltype(vc)::value;
-            a0[v] = (ht)ra[v];
-            a1[v] = (ht)ra[8 + v];
-            b0[v] = (ht)(rb[v] * s);
-            b1[v] = (ht)(rb[8 + v] * s);
+            const float x[4] = {ra[v], ra[8 + v], rb[v] * s, rb[8 + v] * s};
+            ht h[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) h[e] = (ht)x[e];
+            a0[0][v] = h[0];
+            a1[0][v] = h[1];
+            b0[0][v] = h[2];
+            b1[0][v] = h[3];
+            if constexpr (NT == 2) {
+                a0[1][v] = (ht)((x[0] - (float)h[0]) * up);
+                a1[1][v] = (ht)((x[1] - (float)h[1]) * up);
+                b0[1][v] = (ht)((x[2] - (float)h[2]) * up);
+                b1[1][v] = (ht)((x[3] - (float)h[3]) * up);
+            }
         });
-        *reinterpret_cast<hv*>(&As[lr][lk]) = a0;
-        *reinterpret_cast<hv*>(&As[lr][lk + 8]) = a1;
-        *reinterpret_cast<hv*>(&Bs[lr][lk]) = b0;
-        *reinterpret_cast<hv*>(&Bs[lr][lk + 8]) = b1;
+        static_for<0, NT>([&](auto tc) {
+            constexpr int q = decltype(tc)::value;
+            *reinterpret_cast<hv*>(&As[q][lr][lk]) = a0[q];
+            *reinterpret_cast<hv*>(&As[q][lr][lk + 8]) = a1[q];
+            *reinterpret_cast<hv*>(&Bs[q][lr][lk]) = b0[q];
+            *reinterpret_cast<hv*>(&Bs[q][lr][lk + 8]) = b1[q];
+        });
     };
 
     gload(0);
@@ -129,20 +171,28 @@ gemm_lowp_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, 
         lstore();
         __syncthreads();
         if (k0 + LP_BK < k) gload(k0 + LP_BK);    // next tile in flight
-        hv af[4], bf[4];
-        static_for<0, 4>([&](auto ic) {
-            constexpr int i = decltype(ic)::value;
-            af[i] = *reinterpret_cast<const hv*>(&As[wm * 64 + 16 * i + li][8 * g]);
-            bf[i] = *reinterpret_cast<const hv*>(&Bs[wn * 64 + 16 * i + li][8 * g]);
+        hv af[NT][4], bf[NT][4];
+        static_for<0, NT>([&](auto tc) {
+            constexpr int q = decltype(tc)::value;
+            static_for<0, 4>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                af[q][i] = *reinterpret_cast<const hv*>(&As[q][wm * 64 + 16 * i + li][8 * g]);
+                bf[q][i] = *reinterpret_cast<const hv*>(&Bs[q][wn * 64 + 16 * i + li][8 * g]);
+            });
         });
         static_for<0, 4>([&](auto ic) {
             constexpr int i = decltype(ic)::value;
             static_for<0, 4>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
-                acc[i][j] = LowpT<MT>::mac(af[i], bf[j], acc[i][j]);
+                acc[0][i][j] = LT::mac(af[0][i], bf[0][j], acc[0][i][j]);
+                if constexpr (NT == 2) {
+                    acc[NT - 1][i][j] = LT::mac(af[0][i], bf[NT - 1][j], acc[NT - 1][i][j]);
+                    acc[NT - 1][i][j] = LT::mac(af[NT - 1][i], bf[0][j], acc[NT - 1][i][j]);
+                }
             });
         });
     }
+    constexpr float down = 1.0f / (float)(1 << Mode::SHIFT);
     static_for<0, 4>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
         static_for<0, 4>([&](auto jc) {
@@ -153,7 +203,9 @@ gemm_lowp_kernel(int m, int n, int k, float alpha, const float* __restrict__ A, 
                 const int row = r0 + wm * 64 + 16 * i + 4 * g + r;
                 if (row < m && col < n) {
                     float* p = C + row + (i64)col * ldc;
-                    const float v = aeff * acc[i][j][r];
+                    float sum = acc[0][i][j][r];
+                    if constexpr (NT == 2) sum += down * acc[NT - 1][i][j][r];
+                    const float v = aeff * sum;
                     *p = beta == 0.0f ? v : beta * *p + v;
                 }
             });
@@ -191,7 +243,9 @@ int gemm_lowp_launch(int lowprec, i64 m, i64 n, i64 k, float alpha, const float*
     if (m < 0 || n < 0 || k < 0 || m > INT32_MAX || n > INT32_MAX || k > INT32_MAX)
         return TMVX_ERR_ARG;
     if (m == 0 || n == 0) return TMVX_OK;
-    if (lowprec != TMVX_LOWPREC_F16 && lowprec != TMVX_LOWPREC_BF16) return TMVX_ERR_ARG;
+    if (lowprec != TMVX_LOWPREC_F16 && lowprec != TMVX_LOWPREC_BF16 &&
+        lowprec != TMVX_LOWPREC_F16X2 && lowprec != TMVX_LOWPREC_BF16X2)
+        return TMVX_ERR_ARG;
     if (bmax_word) {
         hipLaunchKernelGGL(word_zero_kernel, dim3(1), dim3(1), 0, s, bmax_word);
         const i64 total = k * n;
@@ -203,14 +257,16 @@ int gemm_lowp_launch(int lowprec, i64 m, i64 n, i64 k, float alpha, const float*
     }
     const i64 tiles = ((m + LP_BM - 1) / LP_BM) * ((n + LP_BN - 1) / LP_BN);
     if (tiles > INT32_MAX) return TMVX_ERR_UNSUPPORTED;
-    if (lowprec == TMVX_LOWPREC_F16)
-        hipLaunchKernelGGL(gemm_lowp_kernel<TMVX_LOWPREC_F16>, dim3((unsigned)tiles), dim3(256), 0,
-                           s, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc,
-                           (const unsigned*)bmax_word);
-    else
-        hipLaunchKernelGGL(gemm_lowp_kernel<TMVX_LOWPREC_BF16>, dim3((unsigned)tiles), dim3(256),
-                           0, s, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc,
-                           (const unsigned*)bmax_word);
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(256), 0, s, (int)m, (int)n, (int)k,
+                           alpha, A, lda, B, ldb, beta, C, ldc, (const unsigned*)bmax_word);
+    };
+    switch (lowprec) {
+    case TMVX_LOWPREC_F16: launch(gemm_lowp_kernel<TMVX_LOWPREC_F16>); break;
+    case TMVX_LOWPREC_BF16: launch(gemm_lowp_kernel<TMVX_LOWPREC_BF16>); break;
+    case TMVX_LOWPREC_F16X2: launch(gemm_lowp_kernel<TMVX_LOWPREC_F16X2>); break;
+    default: launch(gemm_lowp_kernel<TMVX_LOWPREC_BF16X2>); break;
+    }
     return hip_status(hipGetLastError());
 }
 
@@ -224,7 +280,9 @@ extern "C" int tmvx_gemm_f32_lowp(int lowprec, int64_t m, int64_t n, int64_t k, 
     if (m < 0 || n < 0 || k < 0 || lda < (m > 0 ? m : 1) || ldb < (k > 0 ? k : 1) ||
         ldc < (m > 0 ? m : 1))
         return TMVX_ERR_ARG;
-    if (lowprec != TMVX_LOWPREC_F16 && lowprec != TMVX_LOWPREC_BF16) return TMVX_ERR_ARG;
+    if (lowprec != TMVX_LOWPREC_F16 && lowprec != TMVX_LOWPREC_BF16 &&
+        lowprec != TMVX_LOWPREC_F16X2 && lowprec != TMVX_LOWPREC_BF16X2)
+        return TMVX_ERR_ARG;
     if (m == 0 || n == 0) return TMVX_OK;
     if (!A || !B || !C) return TMVX_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;

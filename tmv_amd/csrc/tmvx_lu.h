@@ -8,8 +8,9 @@ namespace tmvx {
 // In-place PA = LU (pivots 0-based interchanges).  When perm != nullptr it
 // also receives the composed row permutation: row i of PA is row perm[i] of
 // A.  Synchronises; *info = 1-based first zero pivot or 0.
-// lowprec (FP32 only): TMVX_LOWPREC_F16 / _BF16 run the trailing updates
-// on FP16 / BF16 MFMA with per-block-row scaling (gemm_lowp_launch).
+// lowprec (FP32 only): TMVX_LOWPREC_F16 / _BF16 (/ the split-operand
+// _F16X2 / _BF16X2) run the trailing updates on FP16 / BF16 MFMA with
+// per-block-row scaling (gemm_lowp_launch).
 // pivoting: PIVOT_PARTIAL (TMV's rule), PIVOT_TOURNAMENT (FP32 only,
 // tmvx_tslu.hip) or PIVOT_NONE (no interchanges: piv receives the identity
 // list and *info the first zero or non-finite pivot) -- the last two for the

@@ -597,8 +597,10 @@ int gesv_mixed_f32(int lowprec, i64 n, i64 nrhs, const double* A, i64 lda, const
     if (nrhs > 65535) return TMVX_ERR_UNSUPPORTED;
 
     const int prec = lowprec & 0xf;
+    // GMRES-IR by default where the factors carry 16 bits or fewer
     const bool gmres_first = (lowprec & TMVX_REFINE_GMRES) != 0 ||
-                             (prec != TMVX_LOWPREC_F32 && (lowprec & TMVX_REFINE_CLASSIC) == 0);
+                             (prec != TMVX_LOWPREC_F32 && prec != TMVX_LOWPREC_F16X2 &&
+                              (lowprec & TMVX_REFINE_CLASSIC) == 0);
     const bool try_rbt = (lowprec & TMVX_PIVOT_RBT) != 0;
     // after an RBT factorisation, classical refinement that contracts the
     // error by less than 2x per step hands over to GMRES-IR on the same
@@ -851,7 +853,8 @@ extern "C" int tmvx_gesv_mixed_f64(int lowprec, int64_t n, int64_t nrhs, const d
     const int prec = lowprec & 0xf, flags = lowprec & ~0xf;
     const int refine = flags & (TMVX_REFINE_GMRES | TMVX_REFINE_CLASSIC);
     const int pivot = flags & (TMVX_PIVOT_RBT | TMVX_PIVOT_PARTIAL);
-    if ((prec != TMVX_LOWPREC_F32 && prec != TMVX_LOWPREC_F16 && prec != TMVX_LOWPREC_BF16) ||
+    if ((prec != TMVX_LOWPREC_F32 && prec != TMVX_LOWPREC_F16 && prec != TMVX_LOWPREC_BF16 &&
+         prec != TMVX_LOWPREC_F16X2 && prec != TMVX_LOWPREC_BF16X2) ||
         (flags & ~(TMVX_REFINE_GMRES | TMVX_REFINE_CLASSIC | TMVX_PIVOT_RBT |
                    TMVX_PIVOT_PARTIAL)) != 0 ||
         refine == (TMVX_REFINE_GMRES | TMVX_REFINE_CLASSIC) ||
