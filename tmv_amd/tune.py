@@ -1,12 +1,17 @@
 """GEMM tile-configuration sweep (the tuning tool).
 
     python -m tmv_amd.tune [--n 8192] [--dtype f64|f32|both] [--reps 3]
-    python -m tmv_amd.tune --getrf 16384 [--dtype f32] [--reps 2]
+    python -m tmv_amd.tune --getrf 16384 [--dtype f32] [--reps 2] [--nopiv]
+    python -m tmv_amd.tune --timeline RESULTS.db
 
 Times every configuration of the tile table (include/tmvx_tune.h) on each
 operand layout with HIP events and prints TFLOP/s; --json writes the table.
 --getrf times the blocked LU (uniform[-1,1] matrix of that order) instead,
-for profiling its kernels (panel, interchanges, trailing updates).
+for profiling its kernels (panel, interchanges, trailing updates); with
+--nopiv the LU without interchanges of that matrix plus 2n I.
+--timeline reads a rocprofv3 kernel-trace database (rocpd SQLite) of such
+a run and prints, for the last LU in it (from its getrf_init_kernel on),
+each stream's busy time per kernel family and its idle gaps.
 """
 from __future__ import annotations
 
@@ -82,6 +87,46 @@ def sweep(n=8192, dtypes=("f64", "f32"), reps=3, layouts=("CC", "CR", "RC", "RR"
     return out
 
 
+def _family(name: str) -> str:
+    import re
+    name = re.sub(r"\(.*", "", name)
+    for k in ("swap_solve", "laswp_panel", "panel_kernel", "genp_diag", "genp_l21", "trsm_u12",
+              "getrf_init"):
+        if k in name:
+            return k
+    m = re.search(r"GemmCfg<(\w+), (\d+), (\d+), (\d+)", name)
+    if m:
+        return f"gemm {m.group(1)} {m.group(2)}x{m.group(3)}x{m.group(4)}"
+    return name[:60]
+
+
+def timeline(db_path: str) -> None:
+    """Per-stream busy time by kernel family for the last LU in a rocpd db."""
+    import collections
+    import sqlite3
+    db = sqlite3.connect(db_path)
+    rows = db.execute("select name, stream_id, start, end from kernels order by start").fetchall()
+    starts = [i for i, r in enumerate(rows) if "getrf_init_kernel" in r[0]]
+    if not starts:
+        raise SystemExit("no getrf_init_kernel dispatch in the database")
+    sub = rows[starts[-1]:]
+    t0, t1 = sub[0][2], max(r[3] for r in sub)
+    print(f"LU window {(t1 - t0) / 1e6:.2f} ms, {len(sub)} dispatches")
+    by_stream = collections.defaultdict(list)
+    for r in sub:
+        by_stream[r[1]].append(r)
+    for sid, rs in sorted(by_stream.items()):
+        busy, cnt = collections.Counter(), collections.Counter()
+        for r in rs:
+            busy[_family(r[0])] += (r[3] - r[2]) / 1e6
+            cnt[_family(r[0])] += 1
+        gaps = sum(max(0, rs[i + 1][2] - rs[i][3]) for i in range(len(rs) - 1)) / 1e6
+        print(f"stream {sid}: {len(rs)} kernels, busy {sum(busy.values()):.2f} ms, "
+              f"gaps {gaps:.2f} ms, span {(rs[-1][3] - rs[0][2]) / 1e6:.2f} ms")
+        for k, v in busy.most_common():
+            print(f"    {v:8.2f} ms  {cnt[k]:5d} x {1e3 * v / cnt[k]:8.1f} us  {k}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=8192)
@@ -98,7 +143,14 @@ def main():
                          "(counter calibration)")
     ap.add_argument("--getrf", type=int, default=0,
                     help="instead: time the LU of a uniform matrix of this order")
+    ap.add_argument("--nopiv", action="store_true",
+                    help="with --getrf: the LU without interchanges (of A + 2n I)")
+    ap.add_argument("--timeline", default=None,
+                    help="instead: summarise the last LU of a rocprofv3 kernel-trace db")
     a = ap.parse_args()
+    if a.timeline:
+        timeline(a.timeline)
+        return
     if a.getrf:
         import time
         n = a.getrf
@@ -107,11 +159,14 @@ def main():
         for suf in (("f64", "f32") if a.dtype == "both" else (a.dtype,)):
             dt = torch.float64 if suf == "f64" else torch.float32
             A = A0.to(dt).view(n, n).t()
+            if a.nopiv:
+                idx = torch.arange(n, device="cuda")
+                A[idx, idx] += 2 * n
             for r in range(a.reps):
                 M = A.clone()
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
-                _, info = tmv_amd.getrf(M)
+                _, info = (tmv_amd.getrf_nopiv if a.nopiv else tmv_amd.getrf)(M)
                 torch.cuda.synchronize()
                 print(f"getrf {suf} n={n} rep {r}: {1e3 * (time.perf_counter() - t0):.2f} ms "
                       f"info={info}", flush=True)
