@@ -196,6 +196,9 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb, i64* __restrict_
     __shared__ __attribute__((aligned(16))) T s_wrow[2][NW][BB];   // their block values
     __shared__ __attribute__((aligned(16))) T s_pv[BB];            // the pivot row's block values
     __shared__ int s_p, s_zero, s_fail;
+    // the panel's interchange list, written out at the end: a global store
+    // in the column loop would make the next barrier wait for it
+    __shared__ int s_piv[NB];
     __shared__ int s_own[BB];                    // block pivot i is held here
     // block pivot rows (block order), as 32-bit words: staged by their
     // owners or received; U12 is solved in place in columns >= BB
@@ -416,7 +419,7 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb, i64* __restrict_
                         s_zero = zero ? 1 : 0;
                         s_p = p;
                         if (w == 0) {
-                            piv[j0 + k] = j0 + p;
+                            s_piv[k] = p;
                             if (zero)
                                 atomicMin(reinterpret_cast<unsigned long long*>(wk.first_zero),
                                           (unsigned long long)(j0 + k + 1));
@@ -567,7 +570,9 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb, i64* __restrict_
     }
     // Restore column order (the rotation is by whole blocks).
     while ((nrot * BB) % NB != 0) rotate_blocks();
-    if (!alive || !own) return;
+    if (!alive) return;
+    if (w == 0 && t < jb) piv[j0 + t] = j0 + s_piv[t];
+    if (!own) return;
     T* dst = A + (j0 + pos) + j0 * lda;
 #pragma unroll
     for (int c = 0; c < NB; ++c)
