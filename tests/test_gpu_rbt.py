@@ -93,6 +93,16 @@ def test_mixed_rbt_converges(orc, dev, n, kappa):
         assert fe <= 4 * kappa * max(1e-12, 8 * n * EPS64), (j, fe)
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 13])
+def test_mixed_rbt_tiny(orc, dev, n):
+    """Orders below and around one butterfly group (n padded to 4 or 8)."""
+    A = rng.uniform_matrix(SEED + 50 + n, n, n) + n * np.eye(n)
+    b = rng.uniform(SEED + 60 + n, n, -1.0, 1.0)
+    x, info = tmv_amd.gesv_mixed(to_dev(A), to_dev(b), tol=1e-12, max_iter=40, lowprec=RBT)
+    assert info["converged"] == 1 and info["pivoting"].startswith("rbt"), info
+    np.testing.assert_allclose(to_host(x), orc.gesv(A, b), rtol=0, atol=1e-13)
+
+
 @pytest.mark.parametrize("n", [256, 512])
 def test_mixed_rbt_breakdown_falls_back(dev, n):
     """A = U^-T A'' V^-1 with A''(0,0) = 0: the transformed matrix's first

@@ -48,29 +48,34 @@ adjoint_batched_kernel(int count, const int* __restrict__ nv, const int* __restr
         T* gA = gAv[sys];
         for (int c0 = 0; c0 < nr; c0 += ADJ_RHS) {
             const int cn = (nr - c0) < ADJ_RHS ? (nr - c0) : ADJ_RHS;
-            T gb[ADJ_RHS][WS_PR], xv[ADJ_RHS][WS_PR];
+            // gradB = A^-T G, one right-hand side at a time (a runtime loop:
+            // the solve is inlined once, its vector stays in registers)
+            for (int c = 0; c < cn; ++c) {
+                const i64 off = (i64)(c0 + c) * n;
+                T x[WS_PR];
+                wave_load(x, G + off, n, lane);
+                wave_getrs(1, n, LU, (i64)n, piv, x, lane);
+                wave_store(x, gB + off, n, lane);
+            }
+            // the block's gradB (each lane re-reads what it stored) and X
+            typedef typename WsCol<T>::type V;       // register vectors, not arrays
+            V gb[ADJ_RHS], xv[ADJ_RHS];
             static_for<0, ADJ_RHS>([&](auto cc) __attribute__((always_inline)) {
                 constexpr int c = decltype(cc)::value;
-                if (c < cn) {
-                    const i64 off = (i64)(c0 + c) * n;
-                    wave_load(gb[c], G + off, n, lane);
-                    wave_getrs(1, n, LU, (i64)n, piv, gb[c], lane);
-                    wave_store(gb[c], gB + off, n, lane);
-                    wave_load(xv[c], X + off, n, lane);
-                } else {
-                    static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {
-                        constexpr int s = decltype(sc)::value;
-                        gb[c][s] = T(0);
-                        xv[c][s] = T(0);
-                    });
-                }
+                const i64 off = (i64)(c0 + (c < cn ? c : 0)) * n;
+                static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {
+                    constexpr int s = decltype(sc)::value;
+                    const int i = lane + 64 * s;
+                    gb[c][s] = (c < cn && i < n) ? gB[off + i] : T(0);
+                    xv[c][s] = (c < cn && i < n) ? X[off + i] : T(0);
+                });
             });
             const bool acc = accumulate || c0 > 0;
             for (int k = 0; k < n; ++k) {
                 T xk[ADJ_RHS];
                 static_for<0, ADJ_RHS>([&](auto cc) __attribute__((always_inline)) {
                     constexpr int c = decltype(cc)::value;
-                    xk[c] = wave_get(xv[c], k);
+                    xk[c] = wave_get_v(xv[c], k);
                 });
                 T* col = gA + (i64)k * n;
                 static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {

@@ -99,56 +99,145 @@ __device__ __forceinline__ void wave_swap(T (&x)[WS_PR], int j, int p, int n, in
     });
 }
 
+// Column sweeps with prefetch: BODY runs for the columns k of LU in sweep
+// order (forward 0..n-1 or backward n-1..0) with col[s] = LU(lane + 64 s,
+// k) (rows past n read as 0).  Eight columns are in flight: the loop is
+// unrolled by eight with one named register vector per slot, and a slot's
+// next column (eight steps ahead) is loaded right after the slot is
+// consumed, so a step's loads are issued well before the dependency chain
+// reaches them.  (Named vectors, not an array: an array of buffers used in
+// turn ended up in scratch memory.)
+template <typename T> struct WsCol { typedef T type __attribute__((ext_vector_type(WS_PR))); };
+
+template <typename T>
+__device__ __forceinline__ typename WsCol<T>::type ws_load_col(const T* __restrict__ LU, i64 ld,
+                                                                int n, int q, bool backward,
+                                                                int lane)
+{
+    typename WsCol<T>::type c;
+    const int k = q < n ? (backward ? n - 1 - q : q) : 0;
+    const T* cp = LU + (i64)k * ld;
+    static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {
+        constexpr int s = decltype(sc)::value;
+        const int i = lane + 64 * s;
+        c[s] = (q < n && i < n) ? cp[i] : T(0);
+    });
+    return c;
+}
+
+// Entry k (uniform) of a column vector, in every lane.
+template <typename V>
+__device__ __forceinline__ auto wave_get_v(const V& x, int k)
+{
+    auto v = x[0];
+    static_for<1, WS_PR>([&](auto sc) __attribute__((always_inline)) {
+        constexpr int s = decltype(sc)::value;
+        if ((k >> 6) == s) v = x[s];
+    });
+    return readlane(v, k & 63);
+}
+
+#define TMVX_WS_STEP(SLOT, Q, BACKWARD, ...)                                                 \
+    if ((Q) < n) {                                                                         \
+        const int k = (BACKWARD) ? n - 1 - (Q) : (Q);                                      \
+        const typename WsCol<T>::type col = SLOT;                                          \
+        __VA_ARGS__                                                                        \
+    }                                                                                      \
+    SLOT = ws_load_col(LU, ld, n, (Q) + 8, (BACKWARD), lane);
+#define TMVX_WS_SWEEP(BACKWARD, ...)                                                        \
+    do {                                                                                   \
+        typename WsCol<T>::type p0_ = ws_load_col(LU, ld, n, 0, (BACKWARD), lane);         \
+        typename WsCol<T>::type p1_ = ws_load_col(LU, ld, n, 1, (BACKWARD), lane);         \
+        typename WsCol<T>::type p2_ = ws_load_col(LU, ld, n, 2, (BACKWARD), lane);         \
+        typename WsCol<T>::type p3_ = ws_load_col(LU, ld, n, 3, (BACKWARD), lane);         \
+        typename WsCol<T>::type p4_ = ws_load_col(LU, ld, n, 4, (BACKWARD), lane);         \
+        typename WsCol<T>::type p5_ = ws_load_col(LU, ld, n, 5, (BACKWARD), lane);         \
+        typename WsCol<T>::type p6_ = ws_load_col(LU, ld, n, 6, (BACKWARD), lane);         \
+        typename WsCol<T>::type p7_ = ws_load_col(LU, ld, n, 7, (BACKWARD), lane);         \
+        for (int q0_ = 0; q0_ < n; q0_ += 8) {                                             \
+            TMVX_WS_STEP(p0_, q0_, (BACKWARD), __VA_ARGS__)                                \
+            TMVX_WS_STEP(p1_, q0_ + 1, (BACKWARD), __VA_ARGS__)                            \
+            TMVX_WS_STEP(p2_, q0_ + 2, (BACKWARD), __VA_ARGS__)                            \
+            TMVX_WS_STEP(p3_, q0_ + 3, (BACKWARD), __VA_ARGS__)                            \
+            TMVX_WS_STEP(p4_, q0_ + 4, (BACKWARD), __VA_ARGS__)                            \
+            TMVX_WS_STEP(p5_, q0_ + 5, (BACKWARD), __VA_ARGS__)                            \
+            TMVX_WS_STEP(p6_, q0_ + 6, (BACKWARD), __VA_ARGS__)                            \
+            TMVX_WS_STEP(p7_, q0_ + 7, (BACKWARD), __VA_ARGS__)                            \
+        }                                                                                  \
+    } while (0)
+
+// Interchange list entry j (uniform) from its lane-distributed copy.
+__device__ __forceinline__ int wave_get_int(const int (&v)[WS_PR], int j)
+{
+    int r = v[0];
+    static_for<1, WS_PR>([&](auto sc) __attribute__((always_inline)) {
+        constexpr int s = decltype(sc)::value;
+        if ((j >> 6) == s) r = v[s];
+    });
+    return __builtin_amdgcn_readlane(r, j & 63);
+}
+
 template <typename T>
 __device__ __forceinline__ void wave_getrs(int trans, int n, const T* __restrict__ LU, i64 ld,
                                            const i64* __restrict__ piv, T (&x)[WS_PR], int lane)
 {
+    // the interchange list in registers (one coalesced load, no load on the
+    // chain of swaps)
+    int pv[WS_PR];
+    static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {
+        constexpr int s = decltype(sc)::value;
+        const int i = lane + 64 * s;
+        pv[s] = i < n ? (int)piv[i] : i;
+    });
     if (!trans) {
-        for (int j = 0; j < n; ++j) wave_swap(x, j, (int)piv[j], n, lane);
-        for (int k = 0; k < n; ++k) {                 // L, unit lower
+        for (int j = 0; j < n; ++j) wave_swap(x, j, wave_get_int(pv, j), n, lane);
+        TMVX_WS_SWEEP(false, {                                  // L, unit lower
             const T xk = wave_get(x, k);
             if (xk != T(0)) {
                 static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {
                     constexpr int s = decltype(sc)::value;
                     const int i = lane + 64 * s;
-                    if (i > k && i < n) x[s] -= LU[i + (i64)k * ld] * xk;
+                    if (i > k && i < n) x[s] -= col[s] * xk;
                 });
             }
-        }
-        for (int k = n - 1; k >= 0; --k) {            // U
+        });
+        TMVX_WS_SWEEP(true, {                                   // U
             const T yk = wave_get(x, k);
-            const T xk = yk == T(0) ? T(0) : yk / LU[k + (i64)k * ld];
+            const T xk = yk == T(0) ? T(0) : yk / wave_get_v(col, k);
             wave_set(x, k, xk, lane);
             if (xk != T(0)) {
                 static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {
                     constexpr int s = decltype(sc)::value;
                     const int i = lane + 64 * s;
-                    if (i < k) x[s] -= LU[i + (i64)k * ld] * xk;
+                    if (i < k) x[s] -= col[s] * xk;
                 });
             }
-        }
+        });
     } else {
-        for (int i = 0; i < n; ++i) {                 // U^T: lower, column i of U
+        TMVX_WS_SWEEP(false, {                                  // U^T: column k of U
             T part = T(0);
             static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {
                 constexpr int s = decltype(sc)::value;
-                const int k = lane + 64 * s;
-                if (k < i) part += LU[k + (i64)i * ld] * x[s];
+                const int r = lane + 64 * s;
+                if (r < k) part += col[s] * x[s];
             });
-            const T gi = wave_get(x, i) - wave_sum(part);
-            wave_set(x, i, gi == T(0) ? T(0) : gi / LU[i + (i64)i * ld], lane);
-        }
-        for (int i = n - 1; i >= 0; --i) {            // L^T: unit upper, column i of L
+            const T gk = wave_get(x, k) - wave_sum(part);
+            wave_set(x, k, gk == T(0) ? T(0) : gk / wave_get_v(col, k), lane);
+        });
+        TMVX_WS_SWEEP(true, {                                   // L^T: column k of L
             T part = T(0);
             static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {
                 constexpr int s = decltype(sc)::value;
-                const int k = lane + 64 * s;
-                if (k > i && k < n) part += LU[k + (i64)i * ld] * x[s];
+                const int r = lane + 64 * s;
+                if (r > k && r < n) part += col[s] * x[s];
             });
-            wave_set(x, i, wave_get(x, i) - wave_sum(part), lane);
-        }
-        for (int j = n - 1; j >= 0; --j) wave_swap(x, j, (int)piv[j], n, lane);
+            wave_set(x, k, wave_get(x, k) - wave_sum(part), lane);
+        });
+        for (int j = n - 1; j >= 0; --j) wave_swap(x, j, wave_get_int(pv, j), n, lane);
     }
 }
+
+#undef TMVX_WS_SWEEP
+#undef TMVX_WS_STEP
 
 } // namespace tmvx
