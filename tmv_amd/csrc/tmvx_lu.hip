@@ -1430,81 +1430,115 @@ int getrf_impl(i64 n, T* A, i64 lda, i64* piv, i64* perm, i64* info, hipStream_t
     return getrf_mn_impl<T>(n, n, A, lda, piv, perm, info, s, lowprec, pivoting, na);
 }
 
-// Interchange list -> composed permutation.  Step k swaps positions k and
-// piv[k] >= k, so position k is final after step k.  With the positions'
-// contents in LDS (n <= PERM_LDS_MAX), one wave takes the steps 64 at a
-// time: lane i holds the content of position k0 + i and that of its target
-// piv[k0 + i] when the target lies past the group (lanes sharing a target
-// share the lowest such lane's copy); the 64 swaps run on registers
-// (readlane and selects), the group's positions are then final and written
-// out, and the targets past the group go back to LDS.  Larger n, or a list
-// with some piv[k] < k (not an LU interchange list): one thread, in order.
+// Interchange list -> composed permutation, without a chain of swaps.
+// Step k swaps positions k and p_k >= k, so position k is final after step
+// k, and a position q > k is touched before step k only by the steps j < k
+// with p_j = q (the sources of q).  With f(q) = the content of position q
+// just before step q:
+//   f(q) = f(last source of q), or q when q has no source;
+//   perm[k] = f(k) when p_k = k, else the content of p_k before step k:
+//   f(previous source of p_k), or p_k when k is its first source.
+// One workgroup sorts the keys (p_k, k) of the sources in LDS (bitonic),
+// reads every source's predecessor and every target's last source off the
+// sorted runs, resolves f by pointer jumping (parents point to smaller
+// indices, so ceil(log2 n) rounds suffice; it stops when a round changes
+// nothing) and gathers.  The 8n bytes of perm serve as the workspace: int
+// 2q holds q's parent, int 2k + 1 what perm[k] takes (an index into f, or
+// -(p_k + 1) for the literal p_k), and each thread reads its own pair before
+// it writes perm[k] over it.  n > PERM_LDS_MAX, or a list with some
+// 0 <= piv[k] < k (not an LU interchange list): one thread, in order.
 constexpr int PERM_LDS_MAX = 32768;
+constexpr int PERM_KBITS = 15;                 // k < 2^15 in a key
+static_assert(PERM_LDS_MAX <= (1 << PERM_KBITS), "key packing");
 
 __global__ void __launch_bounds__(1024)
 piv_to_perm_kernel(i64 n, const i64* __restrict__ piv, i64* __restrict__ perm)
 {
-    __shared__ int sp[PERM_LDS_MAX];
-    __shared__ int s_back;               // some piv[k] < k: steps in order, one thread
+    __shared__ int sk[PERM_LDS_MAX];
+    __shared__ int s_back;
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
     const bool lds = n <= PERM_LDS_MAX;
-    if (threadIdx.x == 0) s_back = 0;
+    if (tid == 0) s_back = 0;
     __syncthreads();
-    for (i64 i = threadIdx.x; i < n; i += blockDim.x) {
-        if (lds) sp[i] = (int)i; else perm[i] = i;
-        const i64 p = piv[i];
-        if (p >= 0 && p < i) s_back = 1;
+    int* ws = reinterpret_cast<int*>(perm);
+    const int n32 = lds ? (int)n : 0;
+    int N2 = 1;
+    while (N2 < n32) N2 <<= 1;
+    if (lds) {
+        for (int k = tid; k < N2; k += nt) {
+            int key = INT_MAX;                     // not a source: sorts last
+            if (k < n32) {
+                const i64 p = piv[k];
+                if (p >= 0 && p < k) s_back = 1;
+                if (p > k && p < n) key = ((int)p << PERM_KBITS) | k;
+                ws[2 * k] = k;                     // no source yet: f(k) = k
+                ws[2 * k + 1] = k;                 // p_k = k: perm[k] = f(k)
+            }
+            sk[k] = key;
+        }
+    } else {
+        for (i64 i = tid; i < n; i += nt) {
+            const i64 p = piv[i];
+            if (p >= 0 && p < i) s_back = 1;
+        }
     }
     __syncthreads();
     if (!lds || s_back) {
-        if (threadIdx.x == 0) {
+        for (i64 i = tid; i < n; i += nt) {
+            if (lds) sk[i] = (int)i; else perm[i] = i;
+        }
+        __syncthreads();
+        if (tid == 0) {
             for (i64 k = 0; k < n; ++k) {
                 const i64 p = piv[k];
                 if (p == k || p < 0 || p >= n) continue;
-                if (lds) { const int x = sp[k]; sp[k] = sp[p]; sp[p] = x; }
+                if (lds) { const int x = sk[k]; sk[k] = sk[p]; sk[p] = x; }
                 else { const i64 x = perm[k]; perm[k] = perm[p]; perm[p] = x; }
             }
         }
         __syncthreads();
         if (lds)
-            for (i64 i = threadIdx.x; i < n; i += blockDim.x) perm[i] = sp[i];
+            for (i64 i = tid; i < n; i += nt) perm[i] = sk[i];
         return;
     }
-    if (threadIdx.x >= 64) return;
-    const int lane = threadIdx.x;
-    for (i64 k0 = 0; k0 < n; k0 += 64) {
-        const int cnt = (int)((n - k0) < 64 ? (n - k0) : 64);
-        const i64 k = k0 + lane;
-        const bool act = lane < cnt;
-        i64 pk = act ? piv[k] : k;
-        if (pk < k || pk >= n) pk = k;                    // out of range: no swap
-        const int p = (int)pk;
-        const bool inside = pk < k0 + 64;
-        int cv = act ? sp[k] : 0;                          // content of position k
-        int ov = (act && !inside) ? sp[p] : 0;             // content of target p
-        // owner of an outside target: the lowest lane with the same target
-        int owner = lane;
-        for (int j = 63; j >= 0; --j) {
-            const int pj = __builtin_amdgcn_readlane(p, j);
-            const int oj = __builtin_amdgcn_readlane((int)(act && !inside), j);
-            if (oj && act && !inside && pj == p) owner = j;
-        }
-        for (int i = 0; i < cnt; ++i) {
-            const int pi = __builtin_amdgcn_readlane(p, i);
-            const int a = __builtin_amdgcn_readlane(cv, i);
-            if (pi < k0 + 64) {                            // both positions in the group
-                const int bl = pi - (int)k0;
-                const int b = __builtin_amdgcn_readlane(cv, bl);
-                cv = lane == i ? b : cv;
-                cv = lane == bl ? a : cv;
-            } else {
-                const int o = __builtin_amdgcn_readlane(owner, i);
-                const int b = __builtin_amdgcn_readlane(ov, o);
-                cv = lane == i ? b : cv;
-                ov = lane == o ? a : ov;
+    // bitonic sort of the N2 keys, ascending
+    for (int size = 2; size <= N2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int idx = tid; idx < (N2 >> 1); idx += nt) {
+                const int i = 2 * idx - (idx & (stride - 1));
+                const int a = sk[i], b = sk[i + stride];
+                if ((a > b) == ((i & size) == 0)) { sk[i] = b; sk[i + stride] = a; }
             }
+            __syncthreads();
         }
-        if (act) perm[k] = cv;
-        if (act && !inside && owner == lane) sp[p] = ov;
+    }
+    // runs of one target: predecessor of each source, last source of each target
+    constexpr int KMASK = (1 << PERM_KBITS) - 1;
+    for (int i = tid; i < N2; i += nt) {
+        const int key = sk[i];
+        if (key == INT_MAX) continue;
+        const int t = key >> PERM_KBITS, k = key & KMASK;
+        const int kp = i > 0 ? sk[i - 1] : INT_MAX;
+        const int kn = i + 1 < N2 ? sk[i + 1] : INT_MAX;
+        ws[2 * k + 1] = (kp != INT_MAX && (kp >> PERM_KBITS) == t) ? (kp & KMASK) : -(t + 1);
+        if (kn == INT_MAX || (kn >> PERM_KBITS) != t) ws[2 * t] = k;
+    }
+    __syncthreads();
+    for (int q = tid; q < n32; q += nt) sk[q] = ws[2 * q];
+    __syncthreads();
+    for (int r = 0; (1 << r) < n32; ++r) {     // pointer jumping to the roots
+        int changed = 0;
+        for (int q = tid; q < n32; q += nt) {
+            const int a = sk[q];
+            const int b = sk[a];
+            if (b != a) { sk[q] = b; changed = 1; }
+        }
+        if (!__syncthreads_or(changed)) break;
+    }
+    for (int k = tid; k < n32; k += nt) {
+        const int c = ws[2 * k + 1];
+        perm[k] = c >= 0 ? (i64)sk[c] : (i64)(-(c + 1));
     }
 }
 
