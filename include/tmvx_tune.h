@@ -51,6 +51,22 @@ int tmvx_getrf_nopiv_f32(int64_t n, float* A, int64_t lda, int64_t* piv,
                          int64_t* info, void* stream);
 
 /*
+ * B := op(A)^-1 B for triangular A as the mixed-precision solver's
+ * correction solves compute it: 64 x 64 diagonal blocks applied through
+ * their inverses, formed off the solve's dependency chain (rounding differs
+ * from TMV's substitution by the explicit inverses, whose error grows with
+ * the blocks' condition).  Arguments as tmvx_trsm_f64 / _f32; a zero
+ * diagonal is TMVX_ERR_SINGULAR whatever B holds.  No counterpart in TMV;
+ * exposed for tests and experiments.
+ */
+int tmvx_trsm_blockinv_f64(int upper, int trans, int unit, int64_t n, int64_t nrhs,
+                           const double* A, int64_t lda, double* B, int64_t ldb,
+                           void* stream);
+int tmvx_trsm_blockinv_f32(int upper, int trans, int unit, int64_t n, int64_t nrhs,
+                           const float* A, int64_t lda, float* B, int64_t ldb,
+                           void* stream);
+
+/*
  * Panel timer (diagnostics; no counterpart in TMV).  While started, every
  * panel-kernel launch of the allocating factorisations (tmvx_getrf_*, and
  * through them the solvers) is bracketed by a pair of HIP events recorded

@@ -66,6 +66,8 @@ _SIGS = {
     "tmvx_logdet_f32": (_int, [_i64, _vp, _i64, _vp, ctypes.POINTER(_d), ctypes.POINTER(_d), _vp]),
     "tmvx_trsm_f64": (_int, [_int, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
     "tmvx_trsm_f32": (_int, [_int, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "tmvx_trsm_blockinv_f64": (_int, [_int, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "tmvx_trsm_blockinv_f32": (_int, [_int, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
     "tmvx_laswp_f64": (_int, [_int, _i64, _vp, _i64, _i64, _i64, _vp, _vp]),
     "tmvx_laswp_f32": (_int, [_int, _i64, _vp, _i64, _i64, _i64, _vp, _vp]),
     "tmvx_fill_uniform_f64": (_int, [ctypes.c_uint64, ctypes.c_uint64, _i64, _d, _d, _vp, _vp]),
@@ -577,6 +579,21 @@ def trsm(A, B, upper, trans=False, unit=False, stream=None):
     fn = getattr(lib(), "tmvx_trsm_" + suf)
     _check(fn(int(bool(upper)), int(bool(trans)), int(bool(unit)), n, Bm.shape[1],
               A.data_ptr(), lda, Bm.data_ptr(), ldb, _stream(stream)), "trsm")
+    return B
+
+
+def trsm_blockinv(A, B, upper, trans=False, unit=False, stream=None):
+    """B := op(A)^-1 B with the mixed solver's correction solve (64 x 64
+    diagonal blocks through their inverses; not TMV's rounding).  Raises
+    TmvxError(TMVX_ERR_SINGULAR) on any zero diagonal."""
+    n = _square(A, "A")
+    suf = _suffix(A.dtype)
+    lda = _colmajor_ld(A, "A")
+    Bm, ldb = _rhs(B, n, A.dtype)
+    _on_gpu(A, B)
+    fn = getattr(lib(), "tmvx_trsm_blockinv_" + suf)
+    _check(fn(int(bool(upper)), int(bool(trans)), int(bool(unit)), n, Bm.shape[1],
+              A.data_ptr(), lda, Bm.data_ptr(), ldb, _stream(stream)), "trsm_blockinv")
     return B
 
 

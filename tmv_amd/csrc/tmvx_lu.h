@@ -71,15 +71,19 @@ int trsm_impl(int upper, int trans, int unit, i64 n, i64 nrhs, const T* A, i64 l
 // through out_perm.  X may alias B only when both permutations are null.
 // TA is A's storage type, T the arithmetic and right-hand side type
 // (instantiated for <double, double>, <float, float>, <double, float>).
+// inv = 1: the diagonal blocks are applied through their inverses, off the
+// dependency chain (for the mixed solver's correction solves; a zero
+// diagonal then sets bit 0 of d_singular whatever the right-hand side).
 template <typename T, typename TA = T>
 int trsv_multi(int upper, int trans, int unit, i64 n, i64 nrhs, const TA* A, i64 lda,
                const T* B, i64 ldb, const i64* in_perm, T* X, i64 ldx, const i64* out_perm,
-               int* d_singular, hipStream_t s);
+               int* d_singular, hipStream_t s, int inv = 0);
 
-// Solve with the factors given the composed permutation (see getrf_impl).
+// Solve with the factors given the composed permutation (see getrf_impl);
+// inv as for trsv_multi (used when nrhs <= 16).
 template <typename T>
 int getrs_perm(int trans, i64 n, i64 nrhs, const T* LU, i64 lda, const i64* perm, T* B,
-               i64 ldb, int* d_singular, hipStream_t s);
+               i64 ldb, int* d_singular, hipStream_t s, int inv = 0);
 
 // A side stream with two events for the look-ahead factorisation, taken
 // from a per-device pool (tmvx_util.hip) and returned after use; idle sets

@@ -463,7 +463,11 @@ static int gmres_correction(i64 n, const double* A, i64 lda, const float* LU, co
     // z = M^-1 y with FP64 arithmetic on the FP32-stored factors: applied
     // in FP32 the preconditioner's own error (about kappa n u_32) would
     // leave GMRES a perturbed operator and stall it near 1e-4 for BF16
-    // updates at kappa = 1e4.  After an RBT, M^-1 = V (L U)^-1 U^T.
+    // updates at kappa = 1e4.  After an RBT, M^-1 = V (L U)^-1 U^T.  The
+    // triangular solves apply the diagonal blocks through their inverses
+    // (formed in FP64, off the solves' dependency chain): M^-1 stays one
+    // fixed linear operator, within about u_64 times the blocks'
+    // condition of the substitution.
     auto precond = [&](const double* y, double* z) -> int {
         if (rbt) {
             const i64 N = 4 * rbt->N4;
@@ -472,18 +476,18 @@ static int gmres_correction(i64 n, const double* A, i64 lda, const float* LU, co
                                nullptr, rbt->W, rbt->tmp, N);
             TMVX_CHECK_LAUNCH();
             TMVX_TRY((trsv_multi<double, float>(0, 0, 1, N, 1, LU, N, rbt->tmp, N, nullptr,
-                                                rbt->tmp, N, nullptr, d_flag, s)));
+                                                rbt->tmp, N, nullptr, d_flag, s, 1)));
             TMVX_TRY((trsv_multi<double, float>(1, 0, 0, N, 1, LU, N, rbt->tmp, N, nullptr,
-                                                rbt->tmp, N, nullptr, d_flag, s)));
+                                                rbt->tmp, N, nullptr, d_flag, s, 1)));
             hipLaunchKernelGGL(rbt_right_kernel<double>, g4, b1, 0, s, n, rbt->N4, (i64)1,
                                rbt->tmp, N, nullptr, rbt->W, z, n, 0);
             TMVX_CHECK_LAUNCH();
             return TMVX_OK;
         }
         TMVX_TRY((trsv_multi<double, float>(0, 0, 1, n, 1, LU, n, y, n, perm, z, n, nullptr,
-                                            d_flag, s)));
+                                            d_flag, s, 1)));
         return trsv_multi<double, float>(1, 0, 0, n, 1, LU, n, z, n, nullptr, z, n, nullptr,
-                                         d_flag, s);
+                                         d_flag, s, 1);
     };
     auto matvec = [&](const double* v, double* out) -> int {
         hipLaunchKernelGGL(residual_partial_kernel<1>, dim3(g1.x, g.nchunk), b1, 0, s, n, (i64)1,
@@ -783,22 +787,25 @@ int gesv_mixed_f32(int lowprec, i64 n, i64 nrhs, const double* A, i64 lda, const
                                           &info->inner_iterations, s);
                 if (rc != TMVX_OK) break;
             } else if (rbt) {
-                // d = V (LU)^-1 U^T r / s in FP32;  x += s d
+                // d = V (LU)^-1 U^T r / s in FP32 (inverted diagonal blocks);
+                // x += s d
                 hipLaunchKernelGGL(rbt_left_kernel<float>, g4col, dim3(256), 0, s, n, N4, nrhs,
                                    R, n, d_rmax, Wrbt, R32, N);
                 chk();
                 if (rc == TMVX_OK)
-                    rc = getrs_perm<float>(0, N, nrhs, A32, N, nullptr, R32, N, d_flag, s);
+                    rc = getrs_perm<float>(0, N, nrhs, A32, N, nullptr, R32, N, d_flag, s, 1);
                 if (rc != TMVX_OK) break;
                 hipLaunchKernelGGL(rbt_right_kernel<float>, g4col, dim3(256), 0, s, n, N4, nrhs,
                                    R32, N, d_rmax, Wrbt, X, ldx, 1);
                 chk();
             } else {
-                // d = (LU)^-1 P r / s in FP32;  x += s d
+                // d = (LU)^-1 P r / s in FP32 (diagonal blocks through their
+                // inverses, off the solves' chain);  x += s d
                 hipLaunchKernelGGL(scale_down_kernel, gcol, dim3(256), 0, s, n, nrhs, R, d_rmax,
                                    R32);
                 chk();
-                if (rc == TMVX_OK) rc = getrs_perm<float>(0, n, nrhs, A32, n, perm, R32, n, d_flag, s);
+                if (rc == TMVX_OK)
+                    rc = getrs_perm<float>(0, n, nrhs, A32, n, perm, R32, n, d_flag, s, 1);
                 if (rc != TMVX_OK) break;
                 hipLaunchKernelGGL(update_kernel, gcol, dim3(256), 0, s, n, nrhs, X, ldx, R32,
                                    d_rmax);

@@ -148,7 +148,7 @@ constexpr i64 TRSV_MAX_RHS = TMVX_TRSV_MAX_RHS;
 
 template <typename T>
 int getrs_perm(int trans, i64 n, i64 nrhs, const T* LU, i64 lda, const i64* perm, T* B,
-               i64 ldb, int* d_singular, hipStream_t s)
+               i64 ldb, int* d_singular, hipStream_t s, int inv)
 {
     // A = P^T L U with (P A)[i] = A[perm[i]].
     if (nrhs <= TRSV_MAX_RHS) {
@@ -158,16 +158,16 @@ int getrs_perm(int trans, i64 n, i64 nrhs, const T* LU, i64 lda, const i64* perm
         int rc;
         if (!trans) {
             rc = trsv_multi<T>(0, 0, 1, n, nrhs, LU, lda, B, ldb, perm, W, n, nullptr,
-                               d_singular, s);
+                               d_singular, s, inv);
             if (rc == TMVX_OK)
                 rc = trsv_multi<T>(1, 0, 0, n, nrhs, LU, lda, W, n, nullptr, B, ldb, nullptr,
-                                   d_singular, s);
+                                   d_singular, s, inv);
         } else {
             rc = trsv_multi<T>(1, 1, 0, n, nrhs, LU, lda, B, ldb, nullptr, W, n, nullptr,
-                               d_singular, s);
+                               d_singular, s, inv);
             if (rc == TMVX_OK)
                 rc = trsv_multi<T>(0, 1, 1, n, nrhs, LU, lda, W, n, nullptr, B, ldb, perm,
-                                   d_singular, s);
+                                   d_singular, s, inv);
         }
         hipFreeAsync(W, s);
         return rc;
@@ -270,14 +270,42 @@ int trsm_public(int upper, int trans, int unit, i64 n, i64 nrhs, const T* A, i64
     return rc;
 }
 
+// The mixed solver's correction solve (inverted 64 x 64 diagonal blocks,
+// tmvx_trsv.hip trsv_inv_kernel) on a whole triangle, for tests and
+// experiments: synchronises; a zero diagonal is TMVX_ERR_SINGULAR.
+template <typename T>
+int trsm_blockinv(int upper, int trans, int unit, i64 n, i64 nrhs, const T* A, i64 lda, T* B,
+                  i64 ldb, hipStream_t s)
+{
+    if (n < 0 || nrhs < 0 || lda < (n > 0 ? n : 1) || ldb < (n > 0 ? n : 1))
+        return TMVX_ERR_ARG;
+    if (n == 0 || nrhs == 0) return TMVX_OK;
+    if (!A || !B) return TMVX_ERR_ARG;
+    int* d = nullptr;
+    if (hipMallocAsync((void**)&d, sizeof(int), s) != hipSuccess) return TMVX_ERR_ALLOC;
+    int rc = zero_bytes(d, sizeof(int), s);
+    if (rc == TMVX_OK)
+        rc = trsv_multi<T>(upper, trans, unit, n, nrhs, A, lda, B, ldb, nullptr, B, ldb, nullptr,
+                           d, s, 1);
+    int hs = 0;
+    if (rc == TMVX_OK &&
+        hipMemcpyAsync(&hs, d, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = TMVX_ERR_HIP;
+    hipFreeAsync(d, s);
+    if (hipStreamSynchronize(s) != hipSuccess) return TMVX_ERR_HIP;
+    if (rc == TMVX_OK && (hs & 2)) return TMVX_ERR_TIMEOUT;
+    if (rc == TMVX_OK && hs) return TMVX_ERR_SINGULAR;
+    return rc;
+}
+
 template int trsm_impl<double>(int, int, int, i64, i64, const double*, i64, double*, i64,
                                const i64*, const i64*, int*, hipStream_t);
 template int trsm_impl<float>(int, int, int, i64, i64, const float*, i64, float*, i64,
                               const i64*, const i64*, int*, hipStream_t);
 template int getrs_perm<double>(int, i64, i64, const double*, i64, const i64*, double*, i64,
-                                int*, hipStream_t);
+                                int*, hipStream_t, int);
 template int getrs_perm<float>(int, i64, i64, const float*, i64, const i64*, float*, i64,
-                               int*, hipStream_t);
+                               int*, hipStream_t, int);
 
 } // namespace tmvx
 
@@ -298,6 +326,13 @@ int tmvx_trsm_f64(int upper, int trans, int unit, int64_t n, int64_t nrhs, const
 int tmvx_trsm_f32(int upper, int trans, int unit, int64_t n, int64_t nrhs, const float* A,
                   int64_t lda, float* B, int64_t ldb, void* stream)
 { return tmvx::trsm_public<float>(upper, trans, unit, n, nrhs, A, lda, B, ldb, (hipStream_t)stream); }
+
+int tmvx_trsm_blockinv_f64(int upper, int trans, int unit, int64_t n, int64_t nrhs,
+                           const double* A, int64_t lda, double* B, int64_t ldb, void* stream)
+{ return tmvx::trsm_blockinv<double>(upper, trans, unit, n, nrhs, A, lda, B, ldb, (hipStream_t)stream); }
+int tmvx_trsm_blockinv_f32(int upper, int trans, int unit, int64_t n, int64_t nrhs,
+                           const float* A, int64_t lda, float* B, int64_t ldb, void* stream)
+{ return tmvx::trsm_blockinv<float>(upper, trans, unit, n, nrhs, A, lda, B, ldb, (hipStream_t)stream); }
 
 int tmvx_laswp_f64(int forward, int64_t ncols, double* A, int64_t lda, int64_t k1, int64_t k2,
                    const int64_t* piv, void* stream)

@@ -252,6 +252,241 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
     }
 }
 
+// The same solve with the diagonal substitution taken off the chain, for
+// the mixed-precision solver's correction solves (refinement absorbs the
+// different rounding; the public solves keep the substitution above).
+// With D = op(A)[b, b], A1 = op(A)[b, p] for the block p solved just
+// before b, X = D^-1 and M = X A1 (both formed by the workgroup while
+// earlier blocks are still being solved: X by substitution on the columns
+// of I, 16 columns per wave; M by a 64 x 64 x 64 product):
+//   y   = B_b - sum over the blocks c solved before p of op(A)[b, c] x_c
+//   x_b = X y - M x_p,
+// and X y is formed as soon as y is (one block step before x_p arrives),
+// so the chain carries only x_p's arrival, one 64-term product per lane
+// and the publication of x_b.  Rounding differs from substitution by the
+// explicit inverse (its error grows with the condition of the 64 x 64
+// diagonal blocks, as for BLAS solves that invert diagonal blocks).  A zero
+// diagonal sets bit 0 of `singular` and acts as reciprocal 0.
+template <typename T, typename TA>
+__global__ void __launch_bounds__(256)
+trsv_inv_kernel(int lower, int trans, int unit, i64 n, int nr,
+                const TA* __restrict__ A, i64 lda,
+                const T* __restrict__ Bin, i64 ldb, const i64* __restrict__ in_perm,
+                T* __restrict__ Xout, i64 ldx, const i64* __restrict__ out_perm,
+                u64* __restrict__ xpub, SvWork wk)
+{
+    constexpr int NB = SV_NB, W = SvWords<T>::n, NW = 4, LP = NB + 1, CW = NB / NW;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const i64 nblk = (n + NB - 1) / NB;
+    __shared__ unsigned s_tk;
+    __shared__ int s_fail;
+    __shared__ T s_part[NW][NB];
+    __shared__ T s_d[NB * LP];               // D, then X = D^-1 (row i at i * LP)
+    __shared__ T s_a[NB * LP];               // A1, then M = X A1
+    if (t == 0) { s_tk = atomicAdd(wk.ticket, 1u); s_fail = 0; }
+    __syncthreads();
+    const i64 q = s_tk;                       // position in solve order
+    if (q >= nblk) return;                    // (whole workgroup)
+    const int wpr = nr >= 1 && nr <= NW ? NW / nr : 1;
+    const int rhs = wave / wpr, sub = wave % wpr;
+    const bool active = rhs < nr;
+    const i64 blk = lower ? q : nblk - 1 - q;
+    const i64 b0 = blk * NB;
+    const int bs = (int)((n - b0) < NB ? (n - b0) : NB);
+    const bool has_prev = q > 0;
+    const i64 p0 = (lower ? blk - 1 : blk + 1) * NB;
+    const int ps = has_prev ? (int)((n - p0) < NB ? (n - p0) : NB) : 0;
+    // op(A)(r, c), in A's storage type
+    auto opa = [&](i64 r, i64 c) __attribute__((always_inline)) -> TA {
+        return trans ? A[c + r * lda] : A[r + c * lda];
+    };
+
+    // 1. D (the other triangle zeroed, unit diagonal as 1, identity past
+    // the block's rows) and A1 (zero past bs rows and ps columns) into LDS;
+    // the index order follows the storage so that the loads coalesce.  All
+    // loads are issued before any value is converted and stored, so they
+    // overlap (a conversion next to each load would wait for it).
+    constexpr int PER = NB * NB / 256;
+    {
+        TA dl[PER], al[PER];
+        static_for<0, PER>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            const int e = t + 256 * r;
+            const int i = trans ? (e >> 6) : (e & 63);
+            const int k = trans ? (e & 63) : (e >> 6);
+            dl[r] = (i < bs && k < bs && (lower ? k <= i : k >= i)) ? opa(b0 + i, b0 + k) : TA(0);
+            al[r] = (has_prev && i < bs && k < ps) ? opa(b0 + i, p0 + k) : TA(0);
+        });
+        static_for<0, PER>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            const int e = t + 256 * r;
+            const int i = trans ? (e >> 6) : (e & 63);
+            const int k = trans ? (e & 63) : (e >> 6);
+            T dv = (T)dl[r];
+            if (i == k && (unit || i >= bs)) dv = T(1);
+            s_d[i * LP + k] = dv;
+            if (has_prev) s_a[i * LP + k] = (T)al[r];
+        });
+    }
+    __syncthreads();
+    // 2. X = D^-1, wave w solving D X = I for columns CW w .. CW w + CW-1
+    // (lane i holds row i; step k scales row k by its reciprocal, broadcasts
+    // it and the later rows subtract D(i, k) times it).
+    {
+        const T dii = s_d[lane * LP + lane];
+        const T rdi = dii == T(0) ? T(0) : T(1) / dii;
+        if (wave == 0 && lane < bs && dii == T(0)) atomicOr(wk.singular, 1);
+        T wv[CW];
+        static_for<0, CW>([&](auto jc) __attribute__((always_inline)) {
+            constexpr int jj = decltype(jc)::value;
+            wv[jj] = lane == wave * CW + jj ? T(1) : T(0);
+        });
+        for (int kk = 0; kk < NB; ++kk) {
+            const int k = lower ? kk : NB - 1 - kk;
+            const T dik = s_d[lane * LP + k];
+            const T rk = readlane(rdi, k);
+            static_for<0, CW>([&](auto jc) __attribute__((always_inline)) {
+                constexpr int jj = decltype(jc)::value;
+                const T xk = readlane(wv[jj], k) * rk;
+                wv[jj] = lane == k ? xk : wv[jj] - dik * xk;
+            });
+        }
+        __syncthreads();                      // every wave done with D
+        static_for<0, CW>([&](auto jc) __attribute__((always_inline)) {
+            constexpr int jj = decltype(jc)::value;
+            s_d[lane * LP + wave * CW + jj] = wv[jj];
+        });
+        __syncthreads();
+    }
+    // 3. M = X A1, wave w forming columns CW w .. CW w + CW-1
+    if (has_prev) {
+        T m[CW];
+        static_for<0, CW>([&](auto jc) __attribute__((always_inline)) {
+            m[decltype(jc)::value] = T(0);
+        });
+        for (int k = 0; k < NB; ++k) {
+            const T xik = s_d[lane * LP + k];
+            const T* ak = s_a + k * LP + wave * CW;
+            static_for<0, CW>([&](auto jc) __attribute__((always_inline)) {
+                constexpr int jj = decltype(jc)::value;
+                m[jj] += xik * ak[jj];
+            });
+        }
+        __syncthreads();                      // every wave done with A1
+        static_for<0, CW>([&](auto jc) __attribute__((always_inline)) {
+            constexpr int jj = decltype(jc)::value;
+            s_a[lane * LP + wave * CW + jj] = m[jj];
+        });
+    }
+    __syncthreads();
+
+    // 4. The walk over the blocks solved before p, shared out as in the
+    // substitution kernel; partial sums added in sub-wave order.
+    const bool row_ok = lane < bs;
+    const TA* arow = trans ? A + (b0 + lane) * lda : A + (b0 + lane);
+    const i64 astr = trans ? 1 : lda;
+    const bool lead = active && sub == 0;
+    T acc = T(0);
+    if (lead && row_ok) {
+        i64 src = b0 + lane;
+        if (in_perm) { src = in_perm[b0 + lane]; if (src < 0 || src >= n) src = b0 + lane; }
+        acc = Bin[src + rhs * ldb];
+    }
+    u64* const xcol = xpub + (i64)(active ? rhs : 0) * n * W;
+    bool failed = false;
+    // wait until lanes < cs see the tagged words of x at c0 (bounded)
+    auto wait_x = [&](i64 c0, int cs, T& xc) __attribute__((always_inline)) -> bool {
+        unsigned spins = 0;
+        while (true) {
+            const bool ok = lane >= cs || sv_fetch(xcol + (c0 + lane) * W, xc, wk.epoch);
+            if (__ballot(!ok) == 0ull) return true;
+            if (++spins > SV_SPIN_LIMIT ||
+                ((spins & 255) == 0 &&
+                 __hip_atomic_load(wk.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                if (lane == 0) {
+                    __hip_atomic_store(wk.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    atomicOr(wk.singular, 2);      // bit 1: a bounded wait expired
+                    s_fail = 1;
+                }
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    };
+    for (i64 qc = sub; active && qc + 1 < q && !failed; qc += wpr) {
+        const i64 cb = lower ? qc : nblk - 1 - qc;
+        const i64 c0 = cb * NB;
+        const int cs = (int)((n - c0) < NB ? (n - c0) : NB);
+        TA ablk[NB];                          // storage type: converted at use
+        const TA* ap = arow + c0 * astr;
+        static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = decltype(kc)::value;
+            ablk[k] = (row_ok && k < cs) ? ap[k * astr] : TA(0);
+        });
+        T xc = T(0);
+        if (!wait_x(c0, cs, xc)) { failed = true; break; }
+        T s0 = T(0), s1 = T(0);
+        static_for<0, NB / 2>([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = 2 * decltype(kc)::value;
+            s0 += (T)ablk[k] * readlane(xc, k);
+            s1 += (T)ablk[k + 1] * readlane(xc, k + 1);
+        });
+        acc -= s0 + s1;
+    }
+    if (wpr > 1) {
+        if (active && sub > 0) s_part[wave][lane] = acc;
+        __syncthreads();
+        if (lead)
+            for (int u = 1; u < wpr; ++u) acc += s_part[wave + u][lane];
+    } else {
+        __syncthreads();
+    }
+    if (!lead || s_fail) return;
+    // 5. X y (before x_p arrives; X's row in registers), then
+    // x_b = X y - M x_p
+    T x;
+    {
+        T xrow[NB];
+        static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = decltype(kc)::value;
+            xrow[k] = s_d[lane * LP + k];
+        });
+        T c0 = T(0), c1 = T(0), c2 = T(0), c3 = T(0);
+        static_for<0, NB / 4>([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = 4 * decltype(kc)::value;
+            c0 += xrow[k] * readlane(acc, k);
+            c1 += xrow[k + 1] * readlane(acc, k + 1);
+            c2 += xrow[k + 2] * readlane(acc, k + 2);
+            c3 += xrow[k + 3] * readlane(acc, k + 3);
+        });
+        x = (c0 + c1) + (c2 + c3);
+    }
+    if (has_prev) {
+        T mrow[NB];
+        static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = decltype(kc)::value;
+            mrow[k] = s_a[lane * LP + k];
+        });
+        T xp = T(0);
+        if (!wait_x(p0, ps, xp)) return;
+        T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+        static_for<0, NB / 4>([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = 4 * decltype(kc)::value;
+            s0 += mrow[k] * readlane(xp, k);
+            s1 += mrow[k + 1] * readlane(xp, k + 1);
+            s2 += mrow[k + 2] * readlane(xp, k + 2);
+            s3 += mrow[k + 3] * readlane(xp, k + 3);
+        });
+        x -= (s0 + s1) + (s2 + s3);
+    }
+    if (row_ok) {
+        sv_publish(xcol + (b0 + lane) * W, x, wk.epoch);
+        i64 dst = b0 + lane;
+        if (out_perm) { dst = out_perm[b0 + lane]; if (dst < 0 || dst >= n) dst = b0 + lane; }
+        Xout[dst + rhs * ldx] = x;
+    }
+}
+
 // Per launch: ticket and error flag reset; before the first launch of a
 // solve also the published words (a fresh workspace may hold any tags).
 __global__ void sv_init_kernel(unsigned* ticket, u64* words, i64 nwords, int* err)
@@ -267,7 +502,7 @@ __global__ void sv_init_kernel(unsigned* ticket, u64* words, i64 nwords, int* er
 template <typename T, typename TA>
 int trsv_multi(int upper, int trans, int unit, i64 n, i64 nrhs, const TA* A, i64 lda,
                const T* B, i64 ldb, const i64* in_perm, T* X, i64 ldx, const i64* out_perm,
-               int* d_singular, hipStream_t s)
+               int* d_singular, hipStream_t s, int inv)
 {
     if (n <= 0 || nrhs <= 0) return TMVX_OK;
     const int lower = (upper == 0) != (trans != 0);
@@ -292,9 +527,14 @@ int trsv_multi(int upper, int trans, int unit, i64 n, i64 nrhs, const TA* A, i64
                            dim3(256), 0, s, wk.ticket, xpub, nz, wk.err);
         if ((rc = hip_status(hipGetLastError())) != TMVX_OK) break;
         wk.epoch = epoch;
-        hipLaunchKernelGGL((trsv_persistent_kernel<T, TA>), dim3((unsigned)nblk), dim3(256), 0, s,
-                           lower, trans, unit, n, nr, A, lda, B + j0 * ldb, ldb, in_perm,
-                           X + j0 * ldx, ldx, out_perm, xpub, wk);
+        if (inv)
+            hipLaunchKernelGGL((trsv_inv_kernel<T, TA>), dim3((unsigned)nblk), dim3(256), 0, s,
+                               lower, trans, unit, n, nr, A, lda, B + j0 * ldb, ldb, in_perm,
+                               X + j0 * ldx, ldx, out_perm, xpub, wk);
+        else
+            hipLaunchKernelGGL((trsv_persistent_kernel<T, TA>), dim3((unsigned)nblk), dim3(256),
+                               0, s, lower, trans, unit, n, nr, A, lda, B + j0 * ldb, ldb,
+                               in_perm, X + j0 * ldx, ldx, out_perm, xpub, wk);
         rc = hip_status(hipGetLastError());
     }
     // The timeout flag is reported by the callers that synchronise.
@@ -304,12 +544,12 @@ int trsv_multi(int upper, int trans, int unit, i64 n, i64 nrhs, const TA* A, i64
 
 template int trsv_multi<double, double>(int, int, int, i64, i64, const double*, i64,
                                         const double*, i64, const i64*, double*, i64, const i64*,
-                                        int*, hipStream_t);
+                                        int*, hipStream_t, int);
 template int trsv_multi<float, float>(int, int, int, i64, i64, const float*, i64, const float*,
-                                      i64, const i64*, float*, i64, const i64*, int*, hipStream_t);
+                                      i64, const i64*, float*, i64, const i64*, int*, hipStream_t, int);
 template int trsv_multi<double, float>(int, int, int, i64, i64, const float*, i64,
                                        const double*, i64, const i64*, double*, i64, const i64*,
-                                       int*, hipStream_t);
+                                       int*, hipStream_t, int);
 
 } // namespace tmvx
 
