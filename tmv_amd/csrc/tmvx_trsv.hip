@@ -62,7 +62,9 @@ __device__ __forceinline__ bool sv_fetch(const u64* p, double& x, unsigned epoch
 
 // -DTMVX_TRSV_TRACE: wall-clock marks per block (solve order q), 8 slots:
 // 0 walk reaches block q-1, 1 x_{q-1} seen, 2 partial sums added,
-// 3 diagonal block solved, 4 x_q published, 5 workgroup start.
+// 3 diagonal block solved, 4 x_q published, 5 workgroup start
+// (trsv_inv_kernel: 0 inverse and M formed, 1 partial sums added, 2 X y
+// formed, 3 x_p seen, 4 x_b published, 5 workgroup start).
 #ifdef TMVX_TRSV_TRACE
 __device__ unsigned long long* g_trsv_trace;
 #define TRSV_TRACE(qq, slot)                                                   \
@@ -252,18 +254,50 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
     }
 }
 
+// sum_k row[k] xw[k] for a vector xw in LDS that every lane reads whole
+// (broadcast reads, 16 bytes each): per chunk all reads are issued before
+// the first multiply, so their latencies overlap; four partial sums, k
+// taken in order within each.
+template <typename T, typename R, int NB>
+__device__ __forceinline__ T bcast_dot(const T* __restrict__ xw, const R (&row)[NB])
+{
+    constexpr int VW = 16 / (int)sizeof(T);           // values per 16-byte read
+    constexpr int CH = sizeof(T) == 8 ? NB / 2 : NB;   // values per chunk
+    typedef T VT __attribute__((ext_vector_type(VW)));
+    T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+    static_for<0, NB / CH>([&](auto cc) __attribute__((always_inline)) {
+        constexpr int c0 = decltype(cc)::value * CH;
+        VT xv[CH / VW];
+        static_for<0, CH / VW>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            xv[r] = reinterpret_cast<const VT*>(xw + c0)[r];
+        });
+        __builtin_amdgcn_sched_barrier(0);
+        static_for<0, CH / 4>([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = 4 * decltype(kc)::value;
+            s0 += (T)row[c0 + k] * xv[k / VW][k % VW];
+            s1 += (T)row[c0 + k + 1] * xv[(k + 1) / VW][(k + 1) % VW];
+            s2 += (T)row[c0 + k + 2] * xv[(k + 2) / VW][(k + 2) % VW];
+            s3 += (T)row[c0 + k + 3] * xv[(k + 3) / VW][(k + 3) % VW];
+        });
+    });
+    return (s0 + s1) + (s2 + s3);
+}
+
 // The same solve with the diagonal substitution taken off the chain, for
 // the mixed-precision solver's correction solves (refinement absorbs the
 // different rounding; the public solves keep the substitution above).
-// With D = op(A)[b, b], A1 = op(A)[b, p] for the block p solved just
-// before b, X = D^-1 and M = X A1 (both formed by the workgroup while
-// earlier blocks are still being solved: X by substitution on the columns
-// of I, 16 columns per wave; M by a 64 x 64 x 64 product):
-//   y   = B_b - sum over the blocks c solved before p of op(A)[b, c] x_c
-//   x_b = X y - M x_p,
-// and X y is formed as soon as y is (one block step before x_p arrives),
-// so the chain carries only x_p's arrival, one 64-term product per lane
-// and the publication of x_b.  Rounding differs from substitution by the
+// With D = op(A)[b, b], A1 = op(A)[b, p] and A2 = op(A)[b, p2] for the
+// blocks p and p2 solved one and two before b, X = D^-1, M = X A1 and
+// N = X A2 (formed by the workgroup while earlier blocks are still being
+// solved: X by substitution on the columns of I, 16 columns per wave; M and
+// N by 64 x 64 x 64 products):
+//   y   = B_b - sum over the blocks c solved before p2 of op(A)[b, c] x_c
+//   x_b = X y - N x_p2 - M x_p,
+// X y is formed as soon as y is (two block steps before x_p arrives, so
+// the walk's last wait is off the chain) and N x_p2 one step before, so
+// the chain carries only x_p's arrival, one 64-term product per lane and
+// the publication of x_b.  Rounding differs from substitution by the
 // explicit inverse (its error grows with the condition of the 64 x 64
 // diagonal blocks, as for BLAS solves that invert diagonal blocks).  A zero
 // diagonal sets bit 0 of `singular` and acts as reciprocal 0.
@@ -280,13 +314,18 @@ trsv_inv_kernel(int lower, int trans, int unit, i64 n, int nr,
     const i64 nblk = (n + NB - 1) / NB;
     __shared__ unsigned s_tk;
     __shared__ int s_fail;
-    __shared__ T s_part[NW][NB];
+    // s_part[w]: wave w's broadcast row (a vector one entry per lane, read
+    // back whole by every lane: an LDS broadcast instead of 64 readlanes,
+    // each of which waits for its SGPR), then its partial sums
+    __shared__ __align__(16) T s_part[NW][NB];
     __shared__ T s_d[NB * LP];               // D, then X = D^-1 (row i at i * LP)
     __shared__ T s_a[NB * LP];               // A1, then M = X A1
+    __shared__ T s_n[NB * LP];               // A2, then N = X A2
     if (t == 0) { s_tk = atomicAdd(wk.ticket, 1u); s_fail = 0; }
     __syncthreads();
     const i64 q = s_tk;                       // position in solve order
     if (q >= nblk) return;                    // (whole workgroup)
+    if (wave == 0) TRSV_TRACE(q, 5);
     const int wpr = nr >= 1 && nr <= NW ? NW / nr : 1;
     const int rhs = wave / wpr, sub = wave % wpr;
     const bool active = rhs < nr;
@@ -296,6 +335,9 @@ trsv_inv_kernel(int lower, int trans, int unit, i64 n, int nr,
     const bool has_prev = q > 0;
     const i64 p0 = (lower ? blk - 1 : blk + 1) * NB;
     const int ps = has_prev ? (int)((n - p0) < NB ? (n - p0) : NB) : 0;
+    const bool has_prev2 = q > 1;                 // block solved two before b
+    const i64 p20 = (lower ? blk - 2 : blk + 2) * NB;
+    const int ps2 = has_prev2 ? (int)((n - p20) < NB ? (n - p20) : NB) : 0;
     // op(A)(r, c), in A's storage type
     auto opa = [&](i64 r, i64 c) __attribute__((always_inline)) -> TA {
         return trans ? A[c + r * lda] : A[r + c * lda];
@@ -308,7 +350,7 @@ trsv_inv_kernel(int lower, int trans, int unit, i64 n, int nr,
     // overlap (a conversion next to each load would wait for it).
     constexpr int PER = NB * NB / 256;
     {
-        TA dl[PER], al[PER];
+        TA dl[PER], al[PER], a2[PER];
         static_for<0, PER>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
             const int e = t + 256 * r;
@@ -316,6 +358,7 @@ trsv_inv_kernel(int lower, int trans, int unit, i64 n, int nr,
             const int k = trans ? (e & 63) : (e >> 6);
             dl[r] = (i < bs && k < bs && (lower ? k <= i : k >= i)) ? opa(b0 + i, b0 + k) : TA(0);
             al[r] = (has_prev && i < bs && k < ps) ? opa(b0 + i, p0 + k) : TA(0);
+            a2[r] = (has_prev2 && i < bs && k < ps2) ? opa(b0 + i, p20 + k) : TA(0);
         });
         static_for<0, PER>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
@@ -326,6 +369,7 @@ trsv_inv_kernel(int lower, int trans, int unit, i64 n, int nr,
             if (i == k && (unit || i >= bs)) dv = T(1);
             s_d[i * LP + k] = dv;
             if (has_prev) s_a[i * LP + k] = (T)al[r];
+            if (has_prev2) s_n[i * LP + k] = (T)a2[r];
         });
     }
     __syncthreads();
@@ -358,27 +402,31 @@ trsv_inv_kernel(int lower, int trans, int unit, i64 n, int nr,
         });
         __syncthreads();
     }
-    // 3. M = X A1, wave w forming columns CW w .. CW w + CW-1
-    if (has_prev) {
+    // 3. M = X A1 and N = X A2, wave w forming columns CW w .. CW w + CW-1
+    // (one product at a time: each overwrites its own operand in LDS)
+    auto times_x = [&](T* sm) __attribute__((always_inline)) {
         T m[CW];
         static_for<0, CW>([&](auto jc) __attribute__((always_inline)) {
             m[decltype(jc)::value] = T(0);
         });
         for (int k = 0; k < NB; ++k) {
             const T xik = s_d[lane * LP + k];
-            const T* ak = s_a + k * LP + wave * CW;
+            const T* ak = sm + k * LP + wave * CW;
             static_for<0, CW>([&](auto jc) __attribute__((always_inline)) {
                 constexpr int jj = decltype(jc)::value;
                 m[jj] += xik * ak[jj];
             });
         }
-        __syncthreads();                      // every wave done with A1
+        __syncthreads();                      // every wave done with the operand
         static_for<0, CW>([&](auto jc) __attribute__((always_inline)) {
             constexpr int jj = decltype(jc)::value;
-            s_a[lane * LP + wave * CW + jj] = m[jj];
+            sm[lane * LP + wave * CW + jj] = m[jj];
         });
-    }
+    };
+    if (has_prev) times_x(s_a);
+    if (has_prev2) times_x(s_n);
     __syncthreads();
+    if (wave == 0) TRSV_TRACE(q, 0);
 
     // 4. The walk over the blocks solved before p, shared out as in the
     // substitution kernel; partial sums added in sub-wave order.
@@ -386,6 +434,7 @@ trsv_inv_kernel(int lower, int trans, int unit, i64 n, int nr,
     const TA* arow = trans ? A + (b0 + lane) * lda : A + (b0 + lane);
     const i64 astr = trans ? 1 : lda;
     const bool lead = active && sub == 0;
+    T* const xw = &s_part[wave][0];
     T acc = T(0);
     if (lead && row_ok) {
         i64 src = b0 + lane;
@@ -413,7 +462,7 @@ trsv_inv_kernel(int lower, int trans, int unit, i64 n, int nr,
             __builtin_amdgcn_s_sleep(1);
         }
     };
-    for (i64 qc = sub; active && qc + 1 < q && !failed; qc += wpr) {
+    for (i64 qc = sub; active && qc + 2 < q && !failed; qc += wpr) {
         const i64 cb = lower ? qc : nblk - 1 - qc;
         const i64 c0 = cb * NB;
         const int cs = (int)((n - c0) < NB ? (n - c0) : NB);
@@ -425,13 +474,10 @@ trsv_inv_kernel(int lower, int trans, int unit, i64 n, int nr,
         });
         T xc = T(0);
         if (!wait_x(c0, cs, xc)) { failed = true; break; }
-        T s0 = T(0), s1 = T(0);
-        static_for<0, NB / 2>([&](auto kc) __attribute__((always_inline)) {
-            constexpr int k = 2 * decltype(kc)::value;
-            s0 += (T)ablk[k] * readlane(xc, k);
-            s1 += (T)ablk[k + 1] * readlane(xc, k + 1);
-        });
-        acc -= s0 + s1;
+        xw[lane] = xc;
+        __builtin_amdgcn_wave_barrier();
+        acc -= bcast_dot<T, TA, NB>(xw, ablk);
+        __builtin_amdgcn_wave_barrier();
     }
     if (wpr > 1) {
         if (active && sub > 0) s_part[wave][lane] = acc;
@@ -442,6 +488,7 @@ trsv_inv_kernel(int lower, int trans, int unit, i64 n, int nr,
         __syncthreads();
     }
     if (!lead || s_fail) return;
+    TRSV_TRACE(q, 1);
     // 5. X y (before x_p arrives; X's row in registers), then
     // x_b = X y - M x_p
     T x;
@@ -451,15 +498,24 @@ trsv_inv_kernel(int lower, int trans, int unit, i64 n, int nr,
             constexpr int k = decltype(kc)::value;
             xrow[k] = s_d[lane * LP + k];
         });
-        T c0 = T(0), c1 = T(0), c2 = T(0), c3 = T(0);
-        static_for<0, NB / 4>([&](auto kc) __attribute__((always_inline)) {
-            constexpr int k = 4 * decltype(kc)::value;
-            c0 += xrow[k] * readlane(acc, k);
-            c1 += xrow[k + 1] * readlane(acc, k + 1);
-            c2 += xrow[k + 2] * readlane(acc, k + 2);
-            c3 += xrow[k + 3] * readlane(acc, k + 3);
+        xw[lane] = acc;
+        __builtin_amdgcn_wave_barrier();
+        x = bcast_dot<T, T, NB>(xw, xrow);
+        __builtin_amdgcn_wave_barrier();
+    }
+    TRSV_TRACE(q, 2);
+    if (has_prev2) {
+        T nrow[NB];
+        static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = decltype(kc)::value;
+            nrow[k] = s_n[lane * LP + k];
         });
-        x = (c0 + c1) + (c2 + c3);
+        T xp2 = T(0);
+        if (!wait_x(p20, ps2, xp2)) return;
+        xw[lane] = xp2;
+        __builtin_amdgcn_wave_barrier();
+        x -= bcast_dot<T, T, NB>(xw, nrow);
+        __builtin_amdgcn_wave_barrier();
     }
     if (has_prev) {
         T mrow[NB];
@@ -469,18 +525,14 @@ trsv_inv_kernel(int lower, int trans, int unit, i64 n, int nr,
         });
         T xp = T(0);
         if (!wait_x(p0, ps, xp)) return;
-        T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
-        static_for<0, NB / 4>([&](auto kc) __attribute__((always_inline)) {
-            constexpr int k = 4 * decltype(kc)::value;
-            s0 += mrow[k] * readlane(xp, k);
-            s1 += mrow[k + 1] * readlane(xp, k + 1);
-            s2 += mrow[k + 2] * readlane(xp, k + 2);
-            s3 += mrow[k + 3] * readlane(xp, k + 3);
-        });
-        x -= (s0 + s1) + (s2 + s3);
+        TRSV_TRACE(q, 3);
+        xw[lane] = xp;
+        __builtin_amdgcn_wave_barrier();
+        x -= bcast_dot<T, T, NB>(xw, mrow);
     }
     if (row_ok) {
         sv_publish(xcol + (b0 + lane) * W, x, wk.epoch);
+        TRSV_TRACE(q, 4);
         i64 dst = b0 + lane;
         if (out_perm) { dst = out_perm[b0 + lane]; if (dst < 0 || dst >= n) dst = b0 + lane; }
         Xout[dst + rhs * ldx] = x;
