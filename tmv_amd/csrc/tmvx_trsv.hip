@@ -75,6 +75,36 @@ __device__ unsigned long long* g_trsv_trace;
 #define TRSV_TRACE(qq, slot) do {} while (0)
 #endif
 
+// sum_k row[k] xw[k] for a vector xw in LDS that every lane reads whole
+// (broadcast reads, 16 bytes each): per chunk all reads are issued before
+// the first multiply, so their latencies overlap; four partial sums, k
+// taken in order within each.
+template <typename T, typename R, int NB>
+__device__ __forceinline__ T bcast_dot(const T* __restrict__ xw, const R (&row)[NB])
+{
+    constexpr int VW = 16 / (int)sizeof(T);           // values per 16-byte read
+    constexpr int CH = sizeof(T) == 8 ? NB / 2 : NB;   // values per chunk
+    typedef T VT __attribute__((ext_vector_type(VW)));
+    T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+    static_for<0, NB / CH>([&](auto cc) __attribute__((always_inline)) {
+        constexpr int c0 = decltype(cc)::value * CH;
+        VT xv[CH / VW];
+        static_for<0, CH / VW>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            xv[r] = reinterpret_cast<const VT*>(xw + c0)[r];
+        });
+        __builtin_amdgcn_sched_barrier(0);
+        static_for<0, CH / 4>([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = 4 * decltype(kc)::value;
+            s0 += (T)row[c0 + k] * xv[k / VW][k % VW];
+            s1 += (T)row[c0 + k + 1] * xv[(k + 1) / VW][(k + 1) % VW];
+            s2 += (T)row[c0 + k + 2] * xv[(k + 2) / VW][(k + 2) % VW];
+            s3 += (T)row[c0 + k + 3] * xv[(k + 3) / VW][(k + 3) % VW];
+        });
+    });
+    return (s0 + s1) + (s2 + s3);
+}
+
 struct SvWork {
     unsigned* ticket;     // next block ticket
     int* err;             // timeout
@@ -111,7 +141,7 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
     const i64 nblk = (n + NB - 1) / NB;
     __shared__ unsigned s_tk;
     __shared__ int s_fail;
-    __shared__ T s_part[NW][NB];
+    __shared__ __align__(16) T s_part[NW][NB];   // broadcast row, then partial sums
     if (t == 0) { s_tk = atomicAdd(wk.ticket, 1u); s_fail = 0; }
     __syncthreads();
     const i64 q = s_tk;                       // position in solve order
@@ -187,13 +217,12 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
         }
         if (failed) break;
         if (qc == q - 1) TRSV_TRACE(q, 1);
-        T s0 = T(0), s1 = T(0);
-        static_for<0, NB / 2>([&](auto kc) __attribute__((always_inline)) {
-            constexpr int k = 2 * decltype(kc)::value;
-            s0 += ablk[k] * readlane(xc, k);
-            s1 += ablk[k + 1] * readlane(xc, k + 1);
-        });
-        acc -= s0 + s1;
+        // x_c broadcast through the wave's LDS row (64 readlanes would each
+        // wait for their SGPR)
+        s_part[wave][lane] = xc;
+        __builtin_amdgcn_wave_barrier();
+        acc -= bcast_dot<T, T, NB>(&s_part[wave][0], ablk);
+        __builtin_amdgcn_wave_barrier();
     }
     if (wpr > 1) {                           // partial sums, in sub-wave order
         if (active && sub > 0) s_part[wave][lane] = acc;
@@ -252,36 +281,6 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
         if (out_perm) { dst = out_perm[b0 + lane]; if (dst < 0 || dst >= n) dst = b0 + lane; }
         Xout[dst + rhs * ldx] = x;
     }
-}
-
-// sum_k row[k] xw[k] for a vector xw in LDS that every lane reads whole
-// (broadcast reads, 16 bytes each): per chunk all reads are issued before
-// the first multiply, so their latencies overlap; four partial sums, k
-// taken in order within each.
-template <typename T, typename R, int NB>
-__device__ __forceinline__ T bcast_dot(const T* __restrict__ xw, const R (&row)[NB])
-{
-    constexpr int VW = 16 / (int)sizeof(T);           // values per 16-byte read
-    constexpr int CH = sizeof(T) == 8 ? NB / 2 : NB;   // values per chunk
-    typedef T VT __attribute__((ext_vector_type(VW)));
-    T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
-    static_for<0, NB / CH>([&](auto cc) __attribute__((always_inline)) {
-        constexpr int c0 = decltype(cc)::value * CH;
-        VT xv[CH / VW];
-        static_for<0, CH / VW>([&](auto rc) __attribute__((always_inline)) {
-            constexpr int r = decltype(rc)::value;
-            xv[r] = reinterpret_cast<const VT*>(xw + c0)[r];
-        });
-        __builtin_amdgcn_sched_barrier(0);
-        static_for<0, CH / 4>([&](auto kc) __attribute__((always_inline)) {
-            constexpr int k = 4 * decltype(kc)::value;
-            s0 += (T)row[c0 + k] * xv[k / VW][k % VW];
-            s1 += (T)row[c0 + k + 1] * xv[(k + 1) / VW][(k + 1) % VW];
-            s2 += (T)row[c0 + k + 2] * xv[(k + 2) / VW][(k + 2) % VW];
-            s3 += (T)row[c0 + k + 3] * xv[(k + 3) / VW][(k + 3) % VW];
-        });
-    });
-    return (s0 + s1) + (s2 + s3);
 }
 
 // The same solve with the diagonal substitution taken off the chain, for
