@@ -404,17 +404,14 @@ panel_kernel(T* __restrict__ A, i64 lda, i64 n, i64 j0, int jb, i64* __restrict_
                     const int wl = hg ? (int)__builtin_ctzll(hg) : 0;
                     const bool zero = !(gv >= tiny<T>());              // TMV_Underflow
                     const int p = zero ? k : gi;
-                    // the winner's block values: lane j < BB takes value j
-                    unsigned hv[WPV];
-                    static_for<0, BB>([&](auto jc) {
-                        constexpr int j = decltype(jc)::value;
-#pragma unroll
-                        for (int x = 0; x < WPV; ++x) {
-                            const unsigned u = (unsigned)__builtin_amdgcn_readlane((int)bpay[j * WPV + x], wl);
-                            if (lane == j) hv[x] = u;
-                        }
-                    });
-                    if (lane < BB) s_pv[lane] = from_halves<T>(hv);
+                    // the winner's block values, written by the lane that
+                    // holds them (readlanes would each wait for an SGPR)
+                    if (lane == wl) {
+                        static_for<0, BB>([&](auto jc) {
+                            constexpr int j = decltype(jc)::value;
+                            s_pv[j] = from_halves<T>(&bpay[j * WPV]);
+                        });
+                    }
                     if (lane == 0) {
                         s_zero = zero ? 1 : 0;
                         s_p = p;
