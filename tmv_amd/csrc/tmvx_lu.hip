@@ -594,34 +594,27 @@ __device__ void panel_perm_wave(PanelPerm<NB>& P, const i64* lpiv, i64 j0, int j
     static_assert(NB == 64 || NB == 128, "one or two panel columns per lane");
     constexpr int NPL = NB / 64;
     const int lane = threadIdx.x & 63;
-    long long src[NPL], odst[NPL], osrc[NPL];
+    // The state lives in P (LDS): src by panel row, osrc by outside slot,
+    // read by the whole wave as broadcasts and written by lane 0 (the wave's
+    // LDS accesses are in order, so each step sees the previous one's
+    // stores); odst stays in registers for the ballot search.  (A readlane
+    // per entry would wait for its SGPR on every step.)
+    long long odst[NPL];
 #pragma unroll
-    for (int q = 0; q < NPL; ++q) { src[q] = j0 + lane + 64 * q; odst[q] = -1; osrc[q] = -1; }
-    // entry x of a per-lane array (x uniform): lane x & 63 of chunk x >> 6
-    // (x is uniform: v_readlane, not a cross-lane permute through LDS)
-    auto rl = [](long long v, int l) -> long long {
-        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, l);
-        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
-        return (long long)(((unsigned long long)hi << 32) | lo);
-    };
-    auto get = [&](const long long* v, int x) -> long long {
-        return (NPL > 1 && x >= 64) ? rl(v[NPL - 1], x & 63) : rl(v[0], x & 63);
-    };
-    auto set = [&](long long* v, int x, long long val) {
-        if (lane == (x & 63)) {
-            if (NPL > 1 && x >= 64) v[NPL - 1] = val;
-            else v[0] = val;
-        }
-    };
+    for (int q = 0; q < NPL; ++q) {
+        P.src[lane + 64 * q] = j0 + lane + 64 * q;
+        P.osrc[lane + 64 * q] = -1;
+        odst[q] = -1;
+    }
+    __builtin_amdgcn_wave_barrier();
     int nout = 0;
     for (int k = 0; k < jb; ++k) {
         const i64 p = lpiv[k];
         if (p <= j0 + k || p >= nrows) continue;            // uniform
         if (p < j0 + jb) {
             const int kp = (int)(p - j0);
-            const long long a = get(src, k), b = get(src, kp);
-            set(src, k, b);
-            set(src, kp, a);
+            const i64 a = P.src[k], b = P.src[kp];
+            if (lane == 0) { P.src[k] = b; P.src[kp] = a; }
         } else {
             int q = -1;
 #pragma unroll
@@ -632,20 +625,18 @@ __device__ void panel_perm_wave(PanelPerm<NB>& P, const i64* lpiv, i64 j0, int j
             }
             if (q < 0) {
                 q = nout++;
-                set(odst, q, p);
-                set(osrc, q, p);
+#pragma unroll
+                for (int c = 0; c < NPL; ++c)
+                    if (lane + 64 * c == q) odst[c] = p;
+                if (lane == 0) P.osrc[q] = p;
             }
-            const long long a = get(src, k), b = get(osrc, q);
-            set(src, k, b);
-            set(osrc, q, a);
+            const i64 a = P.src[k], b = P.osrc[q];
+            if (lane == 0) { P.src[k] = b; P.osrc[q] = a; }
         }
+        __builtin_amdgcn_wave_barrier();
     }
 #pragma unroll
-    for (int q = 0; q < NPL; ++q) {
-        P.src[lane + 64 * q] = src[q];
-        P.odst[lane + 64 * q] = odst[q];
-        P.osrc[lane + 64 * q] = osrc[q];
-    }
+    for (int q = 0; q < NPL; ++q) P.odst[lane + 64 * q] = odst[q];
     if (lane == 0) P.nout = nout;
 }
 
