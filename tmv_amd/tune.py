@@ -3,6 +3,8 @@
     python -m tmv_amd.tune [--n 8192] [--dtype f64|f32|both] [--reps 3]
     python -m tmv_amd.tune --getrf 16384 [--dtype f32] [--reps 2] [--nopiv]
     python -m tmv_amd.tune --timeline RESULTS.db
+    python -m tmv_amd.tune --kstats RESULTS.db
+    python -m tmv_amd.tune --trsv 16384 [--dtype f32] [--reps 3]
 
 Times every configuration of the tile table (include/tmvx_tune.h) on each
 operand layout with HIP events and prints TFLOP/s; --json writes the table.
@@ -91,7 +93,7 @@ def _family(name: str) -> str:
     import re
     name = re.sub(r"\(.*", "", name)
     for k in ("swap_solve", "laswp_panel", "panel_kernel", "genp_diag", "genp_l21", "trsm_u12",
-              "getrf_init"):
+              "getrf_init", "trsv_inv", "trsv_persistent"):
         if k in name:
             return k
     m = re.search(r"GemmCfg<(\w+), (\d+), (\d+), (\d+)", name)
@@ -127,6 +129,46 @@ def timeline(db_path: str) -> None:
             print(f"    {v:8.2f} ms  {cnt[k]:5d} x {1e3 * v / cnt[k]:8.1f} us  {k}")
 
 
+def kstats(db_path: str, match: str = "tmvx") -> None:
+    """Per-kernel dispatch statistics (durations end - start) of a rocpd db,
+    kernels whose name contains `match`, by total time."""
+    import collections
+    import sqlite3
+    db = sqlite3.connect(db_path)
+    d = collections.defaultdict(list)
+    for name, s0, s1 in db.execute("select name, start, end from kernels"):
+        if match in name:
+            d[name.split("(")[0].replace("void ", "")].append((s1 - s0) / 1e3)
+    print("| kernel | calls | total ms | avg us | min us | max us |")
+    print("|---|---|---|---|---|---|")
+    for k, v in sorted(d.items(), key=lambda kv: -sum(kv[1])):
+        print(f"| `{k}` | {len(v)} | {sum(v) / 1e3:.2f} | {sum(v) / len(v):.1f} | "
+              f"{min(v):.1f} | {max(v):.1f} |")
+
+
+def trsv_times(n: int, dtypes, reps: int) -> None:
+    """Wall time of the public triangular solve (substitution) and the
+    correction solve (inverted diagonal blocks) on the L and U factors of a
+    uniform matrix's LU, one right-hand side."""
+    import time
+    A0 = torch.empty(n * n, dtype=torch.float64, device="cuda")
+    tmv_amd.fill_uniform(A0, 5)
+    for suf in dtypes:
+        dt = torch.float64 if suf == "f64" else torch.float32
+        LU = A0.to(dt).view(n, n).t()
+        _, info = tmv_amd.getrf(LU)
+        for name, fn in (("substitution", tmv_amd.trsm), ("block inverse", tmv_amd.trsm_blockinv)):
+            for upper in (0, 1):
+                for r in range(reps):
+                    b = torch.ones(n, dtype=dt, device="cuda")
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    fn(LU, b, upper=upper, unit=not upper)
+                    torch.cuda.synchronize()
+                    print(f"trsv {suf} n={n} {name} {'U' if upper else 'L'} rep {r}: "
+                          f"{1e3 * (time.perf_counter() - t0):.3f} ms", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=8192)
@@ -147,9 +189,19 @@ def main():
                     help="with --getrf: the LU without interchanges (of A + 2n I)")
     ap.add_argument("--timeline", default=None,
                     help="instead: summarise the last LU of a rocprofv3 kernel-trace db")
+    ap.add_argument("--kstats", default=None,
+                    help="instead: per-kernel statistics of a rocprofv3 kernel-trace db")
+    ap.add_argument("--trsv", type=int, default=0,
+                    help="instead: time the triangular solves on an LU of this order")
     a = ap.parse_args()
     if a.timeline:
         timeline(a.timeline)
+        return
+    if a.kstats:
+        kstats(a.kstats)
+        return
+    if a.trsv:
+        trsv_times(a.trsv, ("f64", "f32") if a.dtype == "both" else (a.dtype,), a.reps)
         return
     if a.getrf:
         import time
