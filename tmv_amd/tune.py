@@ -4,7 +4,9 @@
     python -m tmv_amd.tune --getrf 16384 [--dtype f32] [--reps 2] [--nopiv]
     python -m tmv_amd.tune --timeline RESULTS.db
     python -m tmv_amd.tune --kstats RESULTS.db
+    python -m tmv_amd.tune --pmcstats RESULTS.db
     python -m tmv_amd.tune --trsv 16384 [--dtype f32] [--reps 3]
+    python -m tmv_amd.tune --batched 20000 [--dtype f64] [--reps 3]
 
 Times every configuration of the tile table (include/tmvx_tune.h) on each
 operand layout with HIP events and prints TFLOP/s; --json writes the table.
@@ -146,6 +148,31 @@ def kstats(db_path: str, match: str = "tmvx") -> None:
               f"{min(v):.1f} | {max(v):.1f} |")
 
 
+def pmcstats(db_path: str, match: str = "tmvx") -> None:
+    """Counter values per dispatch (summed over the device's blocks) of a
+    rocprofv3 --pmc db, kernels whose name contains `match`: the mean over
+    each kernel's dispatches."""
+    import collections
+    import sqlite3
+    db = sqlite3.connect(db_path)
+    q = ("select dispatch_id, kernel_name, counter_name, sum(value), max(duration) "
+         "from counters_collection group by dispatch_id, counter_name")
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    durs = collections.defaultdict(dict)
+    for did, kn, cn, v, du in db.execute(q):
+        if match not in kn:
+            continue
+        k = kn.split("(")[0].replace("void ", "")
+        per[k][cn].append(v)
+        durs[k][did] = du
+    for k, cs in per.items():
+        d = list(durs[k].values())
+        print(f"{k}: {len(d)} dispatches, mean duration {sum(d) / len(d) / 1e3:.1f} us")
+        for cn in sorted(cs):
+            v = cs[cn]
+            print(f"    {cn:24s} {sum(v) / len(v):16.0f}")
+
+
 def trsv_times(n: int, dtypes, reps: int) -> None:
     """Wall time of the public triangular solve (substitution) and the
     correction solve (inverted diagonal blocks) on the L and U factors of a
@@ -167,6 +194,39 @@ def trsv_times(n: int, dtypes, reps: int) -> None:
                     torch.cuda.synchronize()
                     print(f"trsv {suf} n={n} {name} {'U' if upper else 'L'} rep {r}: "
                           f"{1e3 * (time.perf_counter() - t0):.3f} ms", flush=True)
+
+
+def batched_runs(count: int, dtypes, reps: int) -> None:
+    """The batched factor + solve of `count` systems, n ~ U{8..128}, nrhs ~
+    U{1..4}, uniform entries + n on the diagonal (the benchmark's mix), with
+    the default plan, `reps` launches per precision (for counter passes)."""
+    import time
+    import numpy as np
+    g = np.random.default_rng(0x7B)
+    ns = g.integers(8, 129, count).astype(np.int32)
+    nrhs = g.integers(1, 5, count).astype(np.int32)
+    for suf in dtypes:
+        dt = torch.float64 if suf == "f64" else torch.float32
+        b = tmv_amd.Batch(ns, nrhs, dtype=dt)
+        tmp = torch.empty(b.A.numel(), dtype=torch.float64, device="cuda")
+        tmv_amd.fill_uniform(tmp, 0x7C)
+        A0 = tmp.to(dt)
+        diag = np.concatenate([b.offA[i] + np.arange(n) * (n + 1) for i, n in enumerate(ns)])
+        A0[torch.as_tensor(diag, device="cuda")] += torch.as_tensor(
+            np.repeat(ns.astype(np.float64), ns), device="cuda").to(dt)
+        tmp = torch.empty(b.B.numel(), dtype=torch.float64, device="cuda")
+        tmv_amd.fill_uniform(tmp, 0x7D)
+        B0 = tmp.to(dt)
+        plan = tmv_amd.BatchPlan(b)
+        for r in range(reps):
+            b.A.copy_(A0)
+            b.B.copy_(B0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            tmv_amd.gesv_batched(plan, b)
+            torch.cuda.synchronize()
+            print(f"batched {suf} {count} systems rep {r}: "
+                  f"{1e3 * (time.perf_counter() - t0):.3f} ms ({plan.describe()})", flush=True)
 
 
 def main():
@@ -191,8 +251,12 @@ def main():
                     help="instead: summarise the last LU of a rocprofv3 kernel-trace db")
     ap.add_argument("--kstats", default=None,
                     help="instead: per-kernel statistics of a rocprofv3 kernel-trace db")
+    ap.add_argument("--pmcstats", default=None,
+                    help="instead: per-kernel counter means of a rocprofv3 --pmc db")
     ap.add_argument("--trsv", type=int, default=0,
                     help="instead: time the triangular solves on an LU of this order")
+    ap.add_argument("--batched", type=int, default=0,
+                    help="instead: run the batched factor + solve on this many systems")
     a = ap.parse_args()
     if a.timeline:
         timeline(a.timeline)
@@ -200,8 +264,14 @@ def main():
     if a.kstats:
         kstats(a.kstats)
         return
+    if a.pmcstats:
+        pmcstats(a.pmcstats)
+        return
     if a.trsv:
         trsv_times(a.trsv, ("f64", "f32") if a.dtype == "both" else (a.dtype,), a.reps)
+        return
+    if a.batched:
+        batched_runs(a.batched, ("f64", "f32") if a.dtype == "both" else (a.dtype,), a.reps)
         return
     if a.getrf:
         import time
