@@ -9,13 +9,13 @@
 // (x % A, src/TMV_LUDiv.cpp:369-410) and the rank-k update its MultMM
 // (src/TMV_MultMM.cpp:404-477).
 //
-// Batched kernel: one wave per system.  The wave solves A^T gB = G column
-// by column with the stored factors (dot-product sweeps over contiguous
-// columns of U and L), keeps gB and X in registers (lane per row), and
-// writes gradA one column at a time as -sum_c gB(:,c) X(k,c): the rank-k
-// update is fused, so gradA (the n^2 output) is written once and never
-// read unless accumulating.  The recompute variant factors copies of the
-// original matrices first, for callers that did not keep the factors.
+// Batched kernels: one wave per system solves A^T gB = G column by column
+// with the stored factors (dot-product sweeps over contiguous columns of U
+// and L); then many workgroups per system write gradA as
+// -sum_c gB(:,c) X(k,c), 16 columns each (gradA, the n^2 output, is
+// written once and read only when accumulating).  The recompute variant
+// factors copies of the original matrices first, for callers that did not
+// keep the factors.
 #include "tmvx_lu.h"
 #include "tmvx_pivot.h"
 #include "tmvx_wave_solve.h"
@@ -33,8 +33,7 @@ template <typename T>
 __global__ void __launch_bounds__(64)
 adjoint_batched_kernel(int count, const int* __restrict__ nv, const int* __restrict__ nrhsv,
                        const T* const* __restrict__ LUv, const i64* const* __restrict__ pivv,
-                       const T* const* __restrict__ Xv, const T* const* __restrict__ Gv,
-                       T* const* __restrict__ gBv, T* const* __restrict__ gAv, int accumulate)
+                       const T* const* __restrict__ Gv, T* const* __restrict__ gBv)
 {
     const int lane = threadIdx.x & 63;
     for (int sys = blockIdx.x; sys < count; sys += gridDim.x) {
@@ -42,54 +41,56 @@ adjoint_batched_kernel(int count, const int* __restrict__ nv, const int* __restr
         if (n <= 0 || n > WS_NMAX || nr <= 0) continue;
         const T* LU = LUv[sys];
         const i64* piv = pivv[sys];
-        const T* X = Xv[sys];
         const T* G = Gv[sys];
         T* gB = gBv[sys];
+        // gradB = A^-T G, one right-hand side at a time (a runtime loop: the
+        // solve is inlined once, its vector stays in registers)
+        for (int c = 0; c < nr; ++c) {
+            const i64 off = (i64)c * n;
+            T x[WS_PR];
+            wave_load(x, G + off, n, lane);
+            wave_getrs(1, n, LU, (i64)n, piv, x, lane);
+            wave_store(x, gB + off, n, lane);
+        }
+    }
+}
+
+// gradA (+)= -gradB X^T for the systems the wave solves handled, spread
+// over many workgroups (one wave per system writing its n^2 gradient left
+// most of HBM idle): workgroup item (sys, kb) forms columns
+// [ADJ_KC kb, ADJ_KC kb + ADJ_KC) of gradA, one row per thread.  The
+// right-hand sides are taken in blocks of ADJ_RHS, the first block's sum
+// written (or added, when accumulating) and each later block's added, in
+// that order.
+constexpr int ADJ_KC = 16;
+template <typename T>
+__global__ void __launch_bounds__(256)
+adjoint_grad_kernel(int count, const int* __restrict__ nv, const int* __restrict__ nrhsv,
+                    const T* const* __restrict__ Xv, const T* const* __restrict__ gBv,
+                    T* const* __restrict__ gAv, int accumulate)
+{
+    constexpr int NKB = WS_NMAX / ADJ_KC;
+    const i64 items = (i64)count * NKB;
+    for (i64 item = blockIdx.x; item < items; item += gridDim.x) {
+        const int sys = (int)(item / NKB), kb = (int)(item % NKB);
+        const int n = nv[sys], nr = nrhsv[sys];
+        if (n <= 0 || n > WS_NMAX || nr <= 0 || kb * ADJ_KC >= n) continue;
+        const T* X = Xv[sys];
+        const T* gB = gBv[sys];
         T* gA = gAv[sys];
-        for (int c0 = 0; c0 < nr; c0 += ADJ_RHS) {
-            const int cn = (nr - c0) < ADJ_RHS ? (nr - c0) : ADJ_RHS;
-            // gradB = A^-T G, one right-hand side at a time (a runtime loop:
-            // the solve is inlined once, its vector stays in registers)
-            for (int c = 0; c < cn; ++c) {
-                const i64 off = (i64)(c0 + c) * n;
-                T x[WS_PR];
-                wave_load(x, G + off, n, lane);
-                wave_getrs(1, n, LU, (i64)n, piv, x, lane);
-                wave_store(x, gB + off, n, lane);
-            }
-            // the block's gradB (each lane re-reads what it stored) and X
-            typedef typename WsCol<T>::type V;       // register vectors, not arrays
-            V gb[ADJ_RHS], xv[ADJ_RHS];
-            static_for<0, ADJ_RHS>([&](auto cc) __attribute__((always_inline)) {
-                constexpr int c = decltype(cc)::value;
-                const i64 off = (i64)(c0 + (c < cn ? c : 0)) * n;
-                static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {
-                    constexpr int s = decltype(sc)::value;
-                    const int i = lane + 64 * s;
-                    gb[c][s] = (c < cn && i < n) ? gB[off + i] : T(0);
-                    xv[c][s] = (c < cn && i < n) ? X[off + i] : T(0);
-                });
-            });
-            const bool acc = accumulate || c0 > 0;
-            for (int k = 0; k < n; ++k) {
-                T xk[ADJ_RHS];
-                static_for<0, ADJ_RHS>([&](auto cc) __attribute__((always_inline)) {
-                    constexpr int c = decltype(cc)::value;
-                    xk[c] = wave_get_v(xv[c], k);
-                });
-                T* col = gA + (i64)k * n;
-                static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {
-                    constexpr int s = decltype(sc)::value;
-                    const int i = lane + 64 * s;
-                    if (i < n) {
-                        T v = T(0);
-                        static_for<0, ADJ_RHS>([&](auto cc) __attribute__((always_inline)) {
-                            constexpr int c = decltype(cc)::value;
-                            v -= gb[c][s] * xk[c];
-                        });
-                        col[i] = acc ? col[i] + v : v;
-                    }
-                });
+        const int k1 = (kb + 1) * ADJ_KC < n ? (kb + 1) * ADJ_KC : n;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            for (int k = kb * ADJ_KC; k < k1; ++k) {
+                T cur = T(0);
+                for (int c0 = 0; c0 < nr; c0 += ADJ_RHS) {
+                    const int cn = (nr - c0) < ADJ_RHS ? (nr - c0) : ADJ_RHS;
+                    T v = T(0);
+                    for (int c = 0; c < cn; ++c)
+                        v -= gB[i + (i64)(c0 + c) * n] * X[k + (i64)(c0 + c) * n];
+                    if (c0 == 0) cur = accumulate ? gA[i + (i64)k * n] + v : v;
+                    else cur = cur + v;
+                }
+                gA[i + (i64)k * n] = cur;
             }
         }
     }
@@ -124,7 +125,11 @@ static int adjoint_launch(int64_t count, const int32_t* n, const int32_t* nrhs, 
     if (!n || !nrhs || !LU || !piv || !X || !G || !gB || !gA) return TMVX_ERR_ARG;
     const unsigned grid = (unsigned)std::min<int64_t>(count, 4096);
     hipLaunchKernelGGL(adjoint_batched_kernel<T>, dim3(grid), dim3(64), 0, s, (int)count, n, nrhs,
-                       LU, (const i64* const*)piv, X, G, gB, gA, accumulate);
+                       LU, (const i64* const*)piv, G, gB);
+    TMVX_TRY(hip_status(hipGetLastError()));
+    const unsigned grid2 = (unsigned)std::min<int64_t>(count * (WS_NMAX / ADJ_KC), 16384);
+    hipLaunchKernelGGL(adjoint_grad_kernel<T>, dim3(grid2), dim3(256), 0, s, (int)count, n, nrhs,
+                       X, (const T* const*)gB, gA, accumulate);
     return hip_status(hipGetLastError());
 }
 
