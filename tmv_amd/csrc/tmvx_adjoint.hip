@@ -49,7 +49,7 @@ adjoint_batched_kernel(int count, const int* __restrict__ nv, const int* __restr
             const i64 off = (i64)c * n;
             T x[WS_PR];
             wave_load(x, G + off, n, lane);
-            wave_getrs(1, n, LU, (i64)n, piv, x, lane);
+            wave_getrs<T, true>(1, n, LU, (i64)n, piv, x, lane);
             wave_store(x, gB + off, n, lane);
         }
     }

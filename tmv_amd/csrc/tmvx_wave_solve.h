@@ -137,33 +137,72 @@ __device__ __forceinline__ auto wave_get_v(const V& x, int k)
     return readlane(v, k & 63);
 }
 
-#define TMVX_WS_STEP(SLOT, Q, BACKWARD, ...)                                                 \
+#define TMVX_WS_STEP(SLOT, Q, D, BACKWARD, ...)                                              \
     if ((Q) < n) {                                                                         \
         const int k = (BACKWARD) ? n - 1 - (Q) : (Q);                                      \
         const typename WsCol<T>::type col = SLOT;                                          \
         __VA_ARGS__                                                                        \
     }                                                                                      \
-    SLOT = ws_load_col(LU, ld, n, (Q) + 8, (BACKWARD), lane);
+    SLOT = ws_load_col(LU, ld, n, (Q) + (D), (BACKWARD), lane);
 #define TMVX_WS_SWEEP(BACKWARD, ...)                                                        \
-    do {                                                                                   \
-        typename WsCol<T>::type p0_ = ws_load_col(LU, ld, n, 0, (BACKWARD), lane);         \
-        typename WsCol<T>::type p1_ = ws_load_col(LU, ld, n, 1, (BACKWARD), lane);         \
-        typename WsCol<T>::type p2_ = ws_load_col(LU, ld, n, 2, (BACKWARD), lane);         \
-        typename WsCol<T>::type p3_ = ws_load_col(LU, ld, n, 3, (BACKWARD), lane);         \
-        typename WsCol<T>::type p4_ = ws_load_col(LU, ld, n, 4, (BACKWARD), lane);         \
-        typename WsCol<T>::type p5_ = ws_load_col(LU, ld, n, 5, (BACKWARD), lane);         \
-        typename WsCol<T>::type p6_ = ws_load_col(LU, ld, n, 6, (BACKWARD), lane);         \
-        typename WsCol<T>::type p7_ = ws_load_col(LU, ld, n, 7, (BACKWARD), lane);         \
-        for (int q0_ = 0; q0_ < n; q0_ += 8) {                                             \
-            TMVX_WS_STEP(p0_, q0_, (BACKWARD), __VA_ARGS__)                                \
-            TMVX_WS_STEP(p1_, q0_ + 1, (BACKWARD), __VA_ARGS__)                            \
-            TMVX_WS_STEP(p2_, q0_ + 2, (BACKWARD), __VA_ARGS__)                            \
-            TMVX_WS_STEP(p3_, q0_ + 3, (BACKWARD), __VA_ARGS__)                            \
-            TMVX_WS_STEP(p4_, q0_ + 4, (BACKWARD), __VA_ARGS__)                            \
-            TMVX_WS_STEP(p5_, q0_ + 5, (BACKWARD), __VA_ARGS__)                            \
-            TMVX_WS_STEP(p6_, q0_ + 6, (BACKWARD), __VA_ARGS__)                            \
-            TMVX_WS_STEP(p7_, q0_ + 7, (BACKWARD), __VA_ARGS__)                            \
-        }                                                                                  \
+    do {                                                                                    \
+        typename WsCol<T>::type p0_ = ws_load_col(LU, ld, n, 0, (BACKWARD), lane);          \
+        typename WsCol<T>::type p1_ = ws_load_col(LU, ld, n, 1, (BACKWARD), lane);          \
+        typename WsCol<T>::type p2_ = ws_load_col(LU, ld, n, 2, (BACKWARD), lane);          \
+        typename WsCol<T>::type p3_ = ws_load_col(LU, ld, n, 3, (BACKWARD), lane);          \
+        typename WsCol<T>::type p4_ = ws_load_col(LU, ld, n, 4, (BACKWARD), lane);          \
+        typename WsCol<T>::type p5_ = ws_load_col(LU, ld, n, 5, (BACKWARD), lane);          \
+        typename WsCol<T>::type p6_ = ws_load_col(LU, ld, n, 6, (BACKWARD), lane);          \
+        typename WsCol<T>::type p7_ = ws_load_col(LU, ld, n, 7, (BACKWARD), lane);          \
+        for (int q0_ = 0; q0_ < n; q0_ += 8) {                                              \
+            TMVX_WS_STEP(p0_, q0_, 8, (BACKWARD), __VA_ARGS__)                              \
+            TMVX_WS_STEP(p1_, q0_ + 1, 8, (BACKWARD), __VA_ARGS__)                          \
+            TMVX_WS_STEP(p2_, q0_ + 2, 8, (BACKWARD), __VA_ARGS__)                          \
+            TMVX_WS_STEP(p3_, q0_ + 3, 8, (BACKWARD), __VA_ARGS__)                          \
+            TMVX_WS_STEP(p4_, q0_ + 4, 8, (BACKWARD), __VA_ARGS__)                          \
+            TMVX_WS_STEP(p5_, q0_ + 5, 8, (BACKWARD), __VA_ARGS__)                          \
+            TMVX_WS_STEP(p6_, q0_ + 6, 8, (BACKWARD), __VA_ARGS__)                          \
+            TMVX_WS_STEP(p7_, q0_ + 7, 8, (BACKWARD), __VA_ARGS__)                          \
+        }                                                                                   \
+    } while (0)
+// the same with sixteen columns in flight (for callers with registers to
+// spare: the latency of a column load is then spread over sixteen steps)
+#define TMVX_WS_SWEEP16(BACKWARD, ...)                                                      \
+    do {                                                                                    \
+        typename WsCol<T>::type p0_ = ws_load_col(LU, ld, n, 0, (BACKWARD), lane);          \
+        typename WsCol<T>::type p1_ = ws_load_col(LU, ld, n, 1, (BACKWARD), lane);          \
+        typename WsCol<T>::type p2_ = ws_load_col(LU, ld, n, 2, (BACKWARD), lane);          \
+        typename WsCol<T>::type p3_ = ws_load_col(LU, ld, n, 3, (BACKWARD), lane);          \
+        typename WsCol<T>::type p4_ = ws_load_col(LU, ld, n, 4, (BACKWARD), lane);          \
+        typename WsCol<T>::type p5_ = ws_load_col(LU, ld, n, 5, (BACKWARD), lane);          \
+        typename WsCol<T>::type p6_ = ws_load_col(LU, ld, n, 6, (BACKWARD), lane);          \
+        typename WsCol<T>::type p7_ = ws_load_col(LU, ld, n, 7, (BACKWARD), lane);          \
+        typename WsCol<T>::type p8_ = ws_load_col(LU, ld, n, 8, (BACKWARD), lane);          \
+        typename WsCol<T>::type p9_ = ws_load_col(LU, ld, n, 9, (BACKWARD), lane);          \
+        typename WsCol<T>::type p10_ = ws_load_col(LU, ld, n, 10, (BACKWARD), lane);        \
+        typename WsCol<T>::type p11_ = ws_load_col(LU, ld, n, 11, (BACKWARD), lane);        \
+        typename WsCol<T>::type p12_ = ws_load_col(LU, ld, n, 12, (BACKWARD), lane);        \
+        typename WsCol<T>::type p13_ = ws_load_col(LU, ld, n, 13, (BACKWARD), lane);        \
+        typename WsCol<T>::type p14_ = ws_load_col(LU, ld, n, 14, (BACKWARD), lane);        \
+        typename WsCol<T>::type p15_ = ws_load_col(LU, ld, n, 15, (BACKWARD), lane);        \
+        for (int q0_ = 0; q0_ < n; q0_ += 16) {                                             \
+            TMVX_WS_STEP(p0_, q0_, 16, (BACKWARD), __VA_ARGS__)                             \
+            TMVX_WS_STEP(p1_, q0_ + 1, 16, (BACKWARD), __VA_ARGS__)                         \
+            TMVX_WS_STEP(p2_, q0_ + 2, 16, (BACKWARD), __VA_ARGS__)                         \
+            TMVX_WS_STEP(p3_, q0_ + 3, 16, (BACKWARD), __VA_ARGS__)                         \
+            TMVX_WS_STEP(p4_, q0_ + 4, 16, (BACKWARD), __VA_ARGS__)                         \
+            TMVX_WS_STEP(p5_, q0_ + 5, 16, (BACKWARD), __VA_ARGS__)                         \
+            TMVX_WS_STEP(p6_, q0_ + 6, 16, (BACKWARD), __VA_ARGS__)                         \
+            TMVX_WS_STEP(p7_, q0_ + 7, 16, (BACKWARD), __VA_ARGS__)                         \
+            TMVX_WS_STEP(p8_, q0_ + 8, 16, (BACKWARD), __VA_ARGS__)                         \
+            TMVX_WS_STEP(p9_, q0_ + 9, 16, (BACKWARD), __VA_ARGS__)                         \
+            TMVX_WS_STEP(p10_, q0_ + 10, 16, (BACKWARD), __VA_ARGS__)                       \
+            TMVX_WS_STEP(p11_, q0_ + 11, 16, (BACKWARD), __VA_ARGS__)                       \
+            TMVX_WS_STEP(p12_, q0_ + 12, 16, (BACKWARD), __VA_ARGS__)                       \
+            TMVX_WS_STEP(p13_, q0_ + 13, 16, (BACKWARD), __VA_ARGS__)                       \
+            TMVX_WS_STEP(p14_, q0_ + 14, 16, (BACKWARD), __VA_ARGS__)                       \
+            TMVX_WS_STEP(p15_, q0_ + 15, 16, (BACKWARD), __VA_ARGS__)                       \
+        }                                                                                   \
     } while (0)
 
 // Interchange list entry j (uniform) from its lane-distributed copy.
@@ -177,7 +216,7 @@ __device__ __forceinline__ int wave_get_int(const int (&v)[WS_PR], int j)
     return __builtin_amdgcn_readlane(r, j & 63);
 }
 
-template <typename T>
+template <typename T, bool DEEP = false>
 __device__ __forceinline__ void wave_getrs(int trans, int n, const T* __restrict__ LU, i64 ld,
                                            const i64* __restrict__ piv, T (&x)[WS_PR], int lane)
 {
@@ -214,30 +253,39 @@ __device__ __forceinline__ void wave_getrs(int trans, int n, const T* __restrict
             }
         });
     } else {
-        TMVX_WS_SWEEP(false, {                                  // U^T: column k of U
-            T part = T(0);
-            static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {
-                constexpr int s = decltype(sc)::value;
-                const int r = lane + 64 * s;
-                if (r < k) part += col[s] * x[s];
-            });
-            const T gk = wave_get(x, k) - wave_sum(part);
-            wave_set(x, k, gk == T(0) ? T(0) : gk / wave_get_v(col, k), lane);
-        });
-        TMVX_WS_SWEEP(true, {                                   // L^T: column k of L
-            T part = T(0);
-            static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {
-                constexpr int s = decltype(sc)::value;
-                const int r = lane + 64 * s;
-                if (r > k && r < n) part += col[s] * x[s];
-            });
-            wave_set(x, k, wave_get(x, k) - wave_sum(part), lane);
-        });
+        // U^T: column k of U, then L^T: column k of L
+#define TMVX_WS_UT_BODY                                                                    \
+    T part = T(0);                                                                         \
+    static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {                     \
+        constexpr int s = decltype(sc)::value;                                             \
+        const int r = lane + 64 * s;                                                       \
+        if (r < k) part += col[s] * x[s];                                                  \
+    });                                                                                    \
+    const T gk = wave_get(x, k) - wave_sum(part);                                          \
+    wave_set(x, k, gk == T(0) ? T(0) : gk / wave_get_v(col, k), lane);
+#define TMVX_WS_LT_BODY                                                                    \
+    T part = T(0);                                                                         \
+    static_for<0, WS_PR>([&](auto sc) __attribute__((always_inline)) {                     \
+        constexpr int s = decltype(sc)::value;                                             \
+        const int r = lane + 64 * s;                                                       \
+        if (r > k && r < n) part += col[s] * x[s];                                         \
+    });                                                                                    \
+    wave_set(x, k, wave_get(x, k) - wave_sum(part), lane);
+        if constexpr (DEEP) {
+            TMVX_WS_SWEEP16(false, { TMVX_WS_UT_BODY });
+            TMVX_WS_SWEEP16(true, { TMVX_WS_LT_BODY });
+        } else {
+            TMVX_WS_SWEEP(false, { TMVX_WS_UT_BODY });
+            TMVX_WS_SWEEP(true, { TMVX_WS_LT_BODY });
+        }
+#undef TMVX_WS_UT_BODY
+#undef TMVX_WS_LT_BODY
         for (int j = n - 1; j >= 0; --j) wave_swap(x, j, wave_get_int(pv, j), n, lane);
     }
 }
 
 #undef TMVX_WS_SWEEP
+#undef TMVX_WS_SWEEP16
 #undef TMVX_WS_STEP
 
 } // namespace tmvx
