@@ -161,8 +161,10 @@ def test_adjoint_f32(dev):
         assert err <= 64 * 256 * EPS32, float(err)
 
 
-def test_adjoint_accumulate(dev):
-    ns, nrhs = [30, 130], [2, 1]
+@pytest.mark.parametrize("ns,nrhs", [([30, 130], [2, 1]), ([64, 200], [9, 6])])
+def test_adjoint_accumulate(dev, ns, nrhs):
+    """accumulate = 1 adds to gradA (right-hand sides in blocks of four:
+    nrhs 9 and 6 take several blocks)."""
     lay, As, Bs, Ws, Wflat = make(ns, nrhs, SEED + 500)
     fact = tmv_amd.Batch(ns, nrhs)
     fact.A.copy_(lay.A)
