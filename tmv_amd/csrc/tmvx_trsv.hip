@@ -252,25 +252,40 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
         x -= drow[k] * xk;
         if (lane == k) xs = xk;
     };
-    if (lower) {
-        if (bs == NB)
-            static_for<0, NB>([&](auto kc) __attribute__((always_inline)) { dstep(kc); });
-        else
-            static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
-                if (decltype(kc)::value < bs) dstep(kc);
-            });
-    } else {
-        // upper: k = NB-1 .. 0 (rows past bs are skipped)
-        if (bs == NB)
-            static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
-                dstep(std::integral_constant<int, NB - 1 - decltype(kc)::value>{});
-            });
-        else
-            static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
-                constexpr int k = NB - 1 - decltype(kc)::value;
-                if (k < bs) dstep(std::integral_constant<int, k>{});
-            });
-    }
+    // When every reciprocal is finite and non-zero (no zero or subnormal
+    // diagonal entry in the block) the select and the singular test leave
+    // the chain: x_k = x_r (1/d) is then 0 for x_r = 0 as well.
+    auto dstep_fast = [&](auto kc) __attribute__((always_inline)) {
+        constexpr int k = decltype(kc)::value;
+        const T xk = readlane(x, k) * readlane(rdiag, k);
+        x -= drow[k] * xk;
+        if (lane == k) xs = xk;
+    };
+    auto sweep = [&](auto&& step) __attribute__((always_inline)) {
+        if (lower) {
+            if (bs == NB)
+                static_for<0, NB>([&](auto kc) __attribute__((always_inline)) { step(kc); });
+            else
+                static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
+                    if (decltype(kc)::value < bs) step(kc);
+                });
+        } else {
+            // upper: k = NB-1 .. 0 (rows past bs are skipped)
+            if (bs == NB)
+                static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
+                    step(std::integral_constant<int, NB - 1 - decltype(kc)::value>{});
+                });
+            else
+                static_for<0, NB>([&](auto kc) __attribute__((always_inline)) {
+                    constexpr int k = NB - 1 - decltype(kc)::value;
+                    if (k < bs) step(std::integral_constant<int, k>{});
+                });
+        }
+    };
+    const bool fast =
+        __ballot(row_ok && !(rdiag != T(0) && __builtin_isfinite(rdiag))) == 0ull;
+    if (fast) sweep(dstep_fast);
+    else sweep(dstep);
     if (sing) atomicOr(wk.singular, 1);
     TRSV_TRACE(q, 3);
     x = xs;
