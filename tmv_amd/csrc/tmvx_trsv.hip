@@ -190,18 +190,12 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
     }
     u64* const xcol = xpub + (i64)(active ? rhs : 0) * n * W;
     bool failed = false;
-    for (i64 qc = sub; active && qc < q && !failed; qc += wpr) {
-        const i64 cb = lower ? qc : nblk - 1 - qc;
-        const i64 c0 = cb * NB;
-        const int cs = (int)((n - c0) < NB ? (n - c0) : NB);
-        T ablk[NB];
-        load_block(ablk, c0, cs);
-        T xc = T(0);
+    // wait until lanes < cs see the tagged words of x at c0 (bounded)
+    auto wait_x = [&](i64 c0, int cs, T& xc) __attribute__((always_inline)) -> bool {
         unsigned spins = 0;
-        if (qc == q - 1) TRSV_TRACE(q, 0);
         while (true) {
             const bool ok = lane >= cs || sv_fetch(xcol + (c0 + lane) * W, xc, wk.epoch);
-            if (__ballot(!ok) == 0ull) break;
+            if (__ballot(!ok) == 0ull) return true;
             if (++spins > SV_SPIN_LIMIT ||
                 ((spins & 255) == 0 &&
                  __hip_atomic_load(wk.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
@@ -210,20 +204,34 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
                     atomicOr(wk.singular, 2);      // bit 1: a bounded wait expired
                     s_fail = 1;
                 }
-                failed = true;
-                break;
+                return false;
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        if (failed) break;
-        if (qc == q - 1) TRSV_TRACE(q, 1);
-        // x_c broadcast through the wave's LDS row (64 readlanes would each
-        // wait for their SGPR)
+    };
+    // op(A)[b, c] x_c for the finished block at solve position qc, through
+    // the wave's LDS row (64 readlanes would each wait for their SGPR)
+    auto block_product = [&](i64 qc, bool trace) __attribute__((always_inline)) -> bool {
+        const i64 cb = lower ? qc : nblk - 1 - qc;
+        const i64 c0 = cb * NB;
+        const int cs = (int)((n - c0) < NB ? (n - c0) : NB);
+        T ablk[NB];
+        load_block(ablk, c0, cs);
+        T xc = T(0);
+        if (trace) TRSV_TRACE(q, 0);
+        if (!wait_x(c0, cs, xc)) return false;
+        if (trace) TRSV_TRACE(q, 1);
         s_part[wave][lane] = xc;
         __builtin_amdgcn_wave_barrier();
         acc -= bcast_dot<T, T, NB>(&s_part[wave][0], ablk);
         __builtin_amdgcn_wave_barrier();
-    }
+        return true;
+    };
+    // the blocks solved before q - 1, shared out over the sub-waves; the
+    // leading wave takes block q - 1 itself after the partial sums, so the
+    // barrier and the exchange are off the chain
+    for (i64 qc = sub; active && qc + 1 < q && !failed; qc += wpr)
+        failed = !block_product(qc, false);
     if (wpr > 1) {                           // partial sums, in sub-wave order
         if (active && sub > 0) s_part[wave][lane] = acc;
         __syncthreads();
@@ -233,6 +241,7 @@ trsv_persistent_kernel(int lower, int trans, int unit, i64 n, int nr,
         __syncthreads();
     }
     if (!lead || s_fail) return;
+    if (q >= 1 && !block_product(q - 1, true)) return;
     TRSV_TRACE(q, 2);
     // Diagonal block: lane i keeps row i of op(A)[b, b].  Step k: x_k is
     // scaled by the reciprocal of its diagonal (a zero x_k stays as it is;
